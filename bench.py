@@ -1,0 +1,196 @@
+"""Throughput benchmark of the D3Q19 evolve path on one MI355X.
+
+Workload (the headline configuration): lid-driven cavity, 256^3 lattice,
+D3Q19 TRT (magic parameter 3/16), tau = 0.6, rho0 = 1, bounce-back walls on
+all faces, the lid a SOLID layer at the top z plane moving with ux = 0.05.
+A "step" is one fused AA stream+collide pass over the lattice, with the
+populations resident in HBM. The same workload runs in FP64 (the reference's
+numerics), FP32 on shifted populations and FP16 storage / FP32 arithmetic;
+the reduced-precision velocity fields are compared with the FP64 one after
+the same number of steps.
+
+Prints one JSON line (see --help). The CPU baseline is the FP64 CPU oracle
+(a C/OpenMP restatement of the reference algorithm) on a bounded sample of the
+same workload, timed on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "jyu-lb_amd"))
+
+METRIC = "MLUPS D3Q19 AA fp64 vs fp32-shifted; adjoint/forward step time ratio"
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_SITE = {"fp64": 2 * 19 * 8, "fp32": 2 * 19 * 4, "fp16": 2 * 19 * 2}
+KERNEL = {"fp64": "k_step<0,1,*>", "fp32": "k_step<1,1,*>", "fp16": "k_step<2,1,*>"}
+
+
+def cavity_inputs(n):
+    from jyulb.defs import FLUID, SOLID
+    phase = np.full((n, n, n), FLUID, dtype=np.uint8)
+    phase[:, :, n - 1] = SOLID
+    vel = np.zeros((n, n, n, 3))
+    vel[:, :, n - 1, 0] = 0.05
+    den = np.ones((n, n, n))
+    return phase, den, vel
+
+
+def make_cavity(n, precision, layout=0):
+    from jyulb.isothermal import Solver
+    from jyulb.defs import TRT, WALL
+    kvisc = (0.6 - 0.5) / 3.0
+    s = Solver(np.array([n, n, n], dtype=np.int32), TRT, kvisc, np.zeros(3),
+               np.full(6, WALL, dtype=np.int32), precision=precision, layout=layout)
+    phase, den, vel = cavity_inputs(n)
+    s.get_phase()[...] = phase
+    s.get_den()[...] = den
+    s.get_vel()[...] = vel
+    nfl = s.finalize_init()
+    return s, nfl
+
+
+def time_device(solver, steps, warmup):
+    import torch
+    eng = solver.engine
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    eng.evolve(warmup)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    t0.record()
+    eng.evolve(steps)
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    return t0.elapsed_time(t1) / 1e3, wall
+
+
+def cpu_baseline(n, steps):
+    """FP64 CPU oracle on the same cavity (full lattice, `steps` steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import jyulb_oracle as jo
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from jyulb.defs import TRT, WALL
+    s = jo.OracleSolver(np.array([n, n, n]), TRT, (0.6 - 0.5) / 3.0, np.zeros(3), np.full(6, WALL))
+    phase, den, vel = cavity_inputs(n)
+    s.get_phase()[...] = phase
+    s.get_den()[...] = den
+    s.get_vel()[...] = vel
+    nfl = s.finalize_init()
+    t = time.perf_counter()
+    s.evolve(steps)
+    dt = time.perf_counter() - t
+    return dict(value=nfl * steps / dt / 1e6, unit="MLUPS", cores=cores, kind="port",
+                sample=f"FP64 CPU oracle (C/OpenMP restatement of the reference step), same "
+                       f"{n}^3 TRT cavity, {steps} steps after init, {cores} threads, evolve() only")
+
+
+def load_traffic(prec):
+    path = os.path.join(ROOT, "profiles", "c2_cavity_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get(prec, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="cavity edge length")
+    ap.add_argument("--precisions", default="fp64,fp32,fp16")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--layout", type=int, default=0, help="population layout (LB_LAYOUT_*)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not supported "
+              "(the path's slab decomposition is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU available", file=sys.stderr)
+        sys.exit(1)
+    precs = [p for p in args.precisions.split(",") if p]
+    res = {}
+    fields = {}
+    for prec in precs:
+        s, nfl = make_cavity(args.n, prec, args.layout)
+        print(f"[bench] {prec}: {args.n}^3 cavity, {nfl} fluid sites; warmup {args.warmup}, "
+              f"timing {args.steps} steps", file=sys.stderr, flush=True)
+        t, wall = time_device(s, args.steps, args.warmup)
+        den, vel = s.engine.download_den_vel()
+        fields[prec] = vel
+        mlups = nfl * args.steps / t / 1e6
+        gbs = BYTES_PER_SITE[prec] * nfl * args.steps / t / 1e9
+        res[prec] = dict(mlups=mlups, ms_per_step=1e3 * t / args.steps, wall_s=wall, nfluid=nfl,
+                         achieved_gbs=gbs, scale=s.engine.info()["fp16_scale"])
+        print(f"[bench] {prec}: {mlups:.1f} MLUPS, {1e3 * t / args.steps:.4f} ms/step, "
+              f"{gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
+        del s
+        torch.cuda.synchronize()
+    errors = {}
+    if "fp64" in fields:
+        ref = fields["fp64"]
+        fl = np.ones(ref.shape[:3], dtype=bool)
+        fl[:, :, -1] = False
+        r = ref[fl]
+        nref = np.sqrt(np.sum(r * r))
+        for p, v in fields.items():
+            if p == "fp64":
+                continue
+            d = v[fl] - r
+            errors[p] = dict(vel_rel_l2=float(np.sqrt(np.sum(d * d)) / nref),
+                             vel_linf=float(np.max(np.abs(d))),
+                             vel_rel_linf=float(np.max(np.abs(d)) / np.max(np.abs(r))))
+    main_p = "fp64" if "fp64" in res else precs[0]
+    m = res[main_p]
+    cpu = None
+    if not args.no_cpu:
+        print(f"[bench] cpu baseline: {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.n, args.cpu_steps)
+    out = {
+        "metric": METRIC,
+        "value": round(m["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(m["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_p == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": f"lid-driven cavity {args.n}^3, D3Q19 TRT (magic 3/16), tau=0.6, "
+                               f"lid ux=0.05, bounce-back walls", "lattice": [args.n] * 3,
+                   "fluid_sites": m["nfluid"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(m["achieved_gbs"], 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(m["achieved_gbs"] / PEAK_HBM_GBS, 4),
+                     "traffic": load_traffic(main_p), "kernel": KERNEL[main_p],
+                     "bytes_per_site": BYTES_PER_SITE[main_p]},
+        "variants": {p: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                     for p, r in res.items()},
+        "precision_error_vs_fp64": errors,
+        "cpu_baseline": cpu,
+    }
+    if "fp32" in res and "fp64" in res:
+        out["fp32_over_fp64"] = round(res["fp32"]["mlups"] / res["fp64"]["mlups"], 3)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,144 @@
+/*
+ * jyulb_gpu.h -- C ABI of the MI355X D3Q19 evolve engine (libjyulb_hip.so).
+ *
+ * The engine is the device half of the reference's jyulb.isothermal.Solver:
+ * everything that Solver.evolve / finalize_init do per lattice site runs here
+ * in hand-written HIP kernels for gfx950; the Python Solver (jyulb/isothermal.py)
+ * keeps the host-side API, validation and array aliasing and calls these
+ * functions through ctypes. Plain C types only; no torch types.
+ *
+ * Array conventions: host arrays are the *interior* of the reference's
+ * halo-padded fields in C order, i.e. the contiguous copy of
+ * get_phase()/get_den()/get_vel()/get_acc() (isothermal.pyx:1465-1512):
+ *   phase u8 [nx][ny][nz], den f64 [nx][ny][nz], vel/acc f64 [nx][ny][nz][3].
+ * Face data use the reference's halo-padded face shapes (isothermal.pyx:253-263):
+ *   faces 1X/X1: [ny+2][nz+2](,3), 1Y/Y1: [nx+2][nz+2](,3), 1Z/Z1: [nx+2][ny+2](,3).
+ * Population order l = 0..18 as jyulb/D3Q19.pxd:59-79.
+ *
+ * Every function returns LB_OK (0) or an error code; lb_last_error() returns
+ * the message of the last failure on the calling thread. Calls on one handle
+ * must be serialised by the caller (the reference Solver is not re-entrant
+ * either); device work is ordered on the handle's stream.
+ */
+#ifndef JYULB_GPU_H
+#define JYULB_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LB_OK 0
+#define LB_EINVAL 1   /* invalid argument: ValueError in the reference */
+#define LB_ESTATE 2   /* call out of order: RuntimeError in the reference */
+#define LB_EHIP 3     /* HIP runtime failure (no device, launch error, ...) */
+#define LB_ENOMEM 4   /* device allocation failed */
+
+/* Storage / arithmetic variants of the step kernel */
+#define LB_FP64 0          /* f64 populations, f64 arithmetic: the reference's numerics */
+#define LB_FP32_SHIFTED 1  /* f32 storage of f - w*rho0, f32 arithmetic */
+#define LB_FP16_STORAGE 2  /* f16 storage of s*(f - w*rho0), f32 arithmetic */
+
+/* Population layouts in device memory */
+#define LB_LAYOUT_ROWS 0        /* f[i][j][l][k]: the 19 populations of a lattice row side by side */
+#define LB_LAYOUT_SOA 1         /* f[l][i][j][k] */
+#define LB_LAYOUT_SOA_PADDED 2  /* f[l][i][j][k], population stride padded by 2304 bytes */
+
+typedef struct lb_solver lb_solver;
+
+typedef struct {
+    int32_t lat[3];      /* interior lattice size (nx, ny, nz): Solver(lat_size), isothermal.pyx:156 */
+    int32_t rlx_scheme;  /* BGK=0, TRT=1, REGULARIZATION=2 (defs.pxd:86-90) */
+    double kvisc;        /* kinematic viscosity; tau = 0.5 + 3 kvisc (isothermal.pyx:273-287) */
+    double g[3];         /* gravity */
+    int32_t bc[6];       /* per face, WALL=0 PERIODIC=1 FIXED_DEN=2 FIXED_VEL=3 SYMMETRIC=4 */
+    int32_t precision;   /* LB_FP64 / LB_FP32_SHIFTED / LB_FP16_STORAGE */
+    double rho0;         /* reference density of the shifted variants (1.0 if 0) */
+    double fp16_scale;   /* storage scale s of LB_FP16_STORAGE (0: chosen at lb_finalize) */
+    int32_t layout;      /* LB_LAYOUT_* */
+} lb_config;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *lb_last_error(void);
+
+/* Number of HIP devices visible (0 if none); never fails. */
+int lb_device_count(void);
+
+/* Device state of Solver.__cinit__ (isothermal.pyx:156-298): validates the
+ * configuration like _check_bc (isothermal.pyx:378-426) and allocates the
+ * device buffers on the current HIP device. */
+int lb_create(const lb_config *cfg, lb_solver **out);
+int lb_destroy(lb_solver *s);
+
+/* Stream for all device work of this handle (hipStream_t; NULL = default). */
+int lb_set_stream(lb_solver *s, void *hip_stream);
+
+/* gravity setter, isothermal.pyx:369-373 */
+int lb_set_gravity(lb_solver *s, const double g[3]);
+
+/* Interior phase field as written through get_phase() before finalize_init. */
+int lb_upload_phase(lb_solver *s, const uint8_t *phase);
+
+/* den/vel through get_den()/get_vel(): the initial field before lb_finalize,
+ * later the wall velocities of SOLID sites and the densities that Ladd's
+ * moving-wall term reads in the first step of an evolve call
+ * (isothermal.pyx:1117-1119). Device copies are replaced entirely. */
+int lb_upload_den_vel(lb_solver *s, const double *den, const double *vel);
+
+/* Per-site acceleration field (get_acc(), isothermal.pyx:1293-1295);
+ * NULL means the field is identically zero. */
+int lb_upload_acc(lb_solver *s, const double *acc);
+
+/* Zou-He data of one FIXED_DEN/FIXED_VEL face: the arrays that
+ * _set_bc_data copies from den/vel (isothermal.pyx:476-502). */
+int lb_set_face_data(lb_solver *s, int face, const double *den_face, const double *vel_face);
+
+/* finalize_init, device part (isothermal.pyx:1333-1365): link masks from the
+ * phase field (halo passes of _enforce_phase_per_sym_bc folded into address
+ * resolution) and the initial populations of _init_f (isothermal.pyx:1028-1105).
+ * Returns the number of fluid sites in *nfluid. */
+int lb_finalize(lb_solver *s, int64_t *nfluid);
+
+/* The step loop of Solver.evolve (isothermal.pyx:1408-1415): `steps` fused
+ * AA stream+collide steps. den/vel of the fluid sites are stored on the last
+ * step. Asynchronous on the handle's stream. */
+int lb_evolve(lb_solver *s, uint32_t steps);
+
+/* Re-derive the moving-wall flags after lb_upload_den_vel changed wall
+ * velocities (called by the Python Solver before evolve when needed). */
+int lb_refresh_walls(lb_solver *s);
+
+/* den/vel of the last evolve (fluid sites; SOLID sites keep the uploaded values). */
+int lb_download_den_vel(lb_solver *s, double *den, double *vel);
+
+/* Post-collision populations as the reference's f1 after the same number of
+ * steps, unpacked to f64 [19][nx][ny][nz] (entries at SOLID sites are 0). */
+int lb_download_post_f(lb_solver *s, double *f);
+
+/* FP64 reductions over the fluid sites of the last stored den/vel:
+ * out[0] = sum den, out[1..3] = sum vel, out[4..6] = sum den*vel,
+ * out[7] = max |vel|. (The quantities of calc_flow_info,
+ * include/solver/solver.cpp:64-129, and of the examples' mass checks.) */
+int lb_reduce(lb_solver *s, double out[8]);
+
+/* Steady-state test in FP64: ||u - u_prev||_2 / ||u||_2 over the fluid
+ * sites, then u_prev := u. The first call returns 1.0. */
+int lb_convergence(lb_solver *s, double *rel_change);
+
+/* Device-pointer hand-off (e.g. torch data_ptr()): same layouts as the host
+ * variants, pointers in device memory, ordered on the handle's stream. */
+int lb_upload_acc_device(lb_solver *s, const double *acc_dev);
+int lb_download_den_vel_device(lb_solver *s, double *den_dev, double *vel_dev);
+
+/* Introspection: number of AA steps taken, parity of the next step (1 = odd),
+ * bytes of device memory held, storage scale of the FP16 variant. */
+int lb_info(lb_solver *s, uint64_t *steps, int *next_odd, uint64_t *device_bytes, double *fp16_scale);
+
+/* Synchronise the handle's stream. */
+int lb_synchronize(lb_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1161 @@
+// MI355X (gfx950) D3Q19 evolve engine: fused AA-pattern stream+collide
+// kernel, its set-up kernels and the C ABI declared in include/jyulb_gpu.h.
+//
+// Storage. One population array (AA pattern, in place), structure of arrays:
+// population l of interior site (i,j,k) lives at f[l*nq + (i*ny + j)*nzp + k]
+// with the unit-stride axis k (the reference's fastest C-order axis,
+// isothermal.pyx:1532) padded to nzp = round_up(nz, 64) so every row starts on
+// a 128-byte line for all three storage types. No halo layers are stored:
+// the reference's per-step halo passes for periodic and symmetric faces
+// (_enforce_distr_per_sym_bc, isothermal.pyx:691-837) are folded into
+// address resolution, and WALL / FIXED faces are SOLID halos, i.e. bounce-back
+// links recorded in a per-site link mask.
+//
+// AA pattern. After an "even" step site x holds its post-collision f*_l in
+// its own slot opp(l). An odd step pulls f_l(x) = f*_{l'}(y') from slot
+// opp(l') of the resolved upstream site (y',l') = resolve(x - c_l, l), or from
+// its own slot l for a bounce-back link, collides, and writes f*_{opp(l)}(x)
+// back into the very location it read f_l from. An even step reads its own
+// slots l and writes f*_l into its own slot opp(l). Every location is read
+// and written by one thread only, so one array suffices (the C++ exe of the
+// reference uses the same scheme with a tabulated index map,
+// include/kernel/kernel.cpp:94-139, 469-552; here addresses are computed).
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <math.h>
+#include <stdarg.h>
+#include <algorithm>
+#include <utility>
+#include <vector>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+
+#include "../../include/jyulb_gpu.h"
+#include "lb_lattice.h"
+#include "lb_physics.h"
+
+using namespace lb;
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(LB_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// storage variants
+// ---------------------------------------------------------------------------
+template <int P> struct Var;
+template <> struct Var<LB_FP64> { using TS = double; using TC = double; static constexpr bool SH = false; };
+template <> struct Var<LB_FP32_SHIFTED> { using TS = float; using TC = float; static constexpr bool SH = true; };
+template <> struct Var<LB_FP16_STORAGE> { using TS = __half; using TC = float; static constexpr bool SH = true; };
+
+template <class TC> LBD TC ldp(const double* p) { return (TC)(*p); }
+
+template <int P>
+LBD typename Var<P>::TC ldf(const typename Var<P>::TS* p, float inv_scale) {
+    if constexpr (P == LB_FP16_STORAGE) return __half2float(*p) * inv_scale;
+    else { (void)inv_scale; return *p; }
+}
+template <int P>
+LBD void stf(typename Var<P>::TS* p, typename Var<P>::TC v, float scale) {
+    if constexpr (P == LB_FP16_STORAGE) *p = __float2half_rn(v * scale);
+    else { (void)scale; *p = v; }
+}
+
+// ---------------------------------------------------------------------------
+// geometry helpers
+// ---------------------------------------------------------------------------
+struct Dims {
+    int nx, ny, nz, nzp;
+    int bc[6];
+};
+
+__constant__ int8_t c_mirror[3][Q];
+
+// Resolve a possibly out-of-range site (i,j,k) reached along lattice vector l
+// to the interior site whose data it carries, undoing the reference's halo
+// passes in reverse order (z, then y, then x: the x pass covers interior j,k
+// only, the y pass all i and interior k, the z pass all i and j,
+// isothermal.pyx:509-837). Periodic faces wrap, symmetric faces mirror the
+// site onto the adjacent layer and reflect l and the velocity component
+// (flip bit per axis). Returns false for a WALL/FIXED halo: a SOLID site at
+// rest.
+LBD bool resolve(const Dims& d, int& i, int& j, int& k, int& l, int& flip) {
+    int c[3] = {i, j, k};
+    const int n[3] = {d.nx, d.ny, d.nz};
+#pragma unroll
+    for (int a = 2; a >= 0; --a) {
+        if (c[a] < 0 || c[a] >= n[a]) {
+            const bool lo = c[a] < 0;
+            const int t = d.bc[2 * a + (lo ? 0 : 1)];
+            if (t == PERIODIC) {
+                c[a] += lo ? n[a] : -n[a];
+            } else if (t == SYMMETRIC) {
+                c[a] = lo ? 0 : n[a] - 1;
+                l = c_mirror[a][l];
+                flip ^= 1 << a;
+            } else {
+                return false;
+            }
+        }
+    }
+    i = c[0];
+    j = c[1];
+    k = c[2];
+    return true;
+}
+
+LBD void decompose(uint32_t c, int ny, int nzp, int& i, int& j, int& k) {
+    uint32_t row = c / (uint32_t)nzp;
+    k = (int)(c - row * (uint32_t)nzp);
+    i = (int)(row / (uint32_t)ny);
+    j = (int)(row - (uint32_t)i * (uint32_t)ny);
+}
+
+// blockIdx -> tile map that hands each XCD a contiguous range of tiles
+// (workgroups are dealt to the 8 XCDs round-robin); gridDim.x % 8 == 0.
+LBD uint32_t xcd_tile() {
+    const uint32_t per = gridDim.x >> 3;
+    return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// step kernel
+// ---------------------------------------------------------------------------
+struct StepParams {
+    void* f;
+    const uint32_t* mask;
+    double* den;
+    double* vel;  // 3 * nq, SoA
+    const void* acc;  // 3 * nq of the compute type, SoA, or null
+    const double* fden[6];
+    const double* fvel[6];
+    double g[3];
+    double prm[2];
+    double rho0;
+    float scale, inv_scale;
+    Dims d;
+    int NJ, NK;  // halo-padded sizes (face-data indexing)
+    uint32_t nq;
+    uint32_t ntiles;
+    int store_dv;
+    int moving;
+    int64_t rs;  // element stride between rows (i,j) of the population array
+    int64_t ps;  // element stride between populations
+};
+
+// expanded index of padded site c in the population array: population l of
+// site (i,j,k) lives at (i*ny + j)*rs + k + l*ps
+LBD int64_t eidx(const StepParams& p, uint32_t c) {
+    if (p.rs == p.d.nzp) return c;
+    const uint32_t row = c / (uint32_t)p.d.nzp;
+    return (int64_t)row * p.rs + (c - row * (uint32_t)p.d.nzp);
+}
+
+constexpr int TILE = 256;
+
+// element offset of the location an odd step reads f_l from (and writes
+// f*_opp(l) back to)
+LBD int64_t odd_loc(const StepParams& p, uint32_t m, int64_t e, int l, int i, int j, int k) {
+    if (l == REST) return e + (int64_t)REST * p.ps;
+    if (m & (1u << l)) return e + (int64_t)l * p.ps;
+    if (!(m & M_BND))
+        return e - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs - CK[l] + (int64_t)opp(l) * p.ps;
+    int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
+    resolve(p.d, yi, yj, yk, lp, fl);
+    return ((int64_t)yi * p.d.ny + yj) * p.rs + yk + (int64_t)(18 - lp) * p.ps;
+}
+
+// Ladd's moving-wall term for bounce-back link l (isothermal.pyx:1112-1248):
+// f_l = f_opp +/- cff*(u_a +/- u_b), u the velocity of the solid site
+// x - c_l (mirrored through symmetric faces, zero for WALL/FIXED halos).
+template <class T, int L>
+LBD void ladd(const StepParams& p, T* f, T cff1, T cff2, int i, int j, int k) {
+    int yi = i - CI[L], yj = j - CJ[L], yk = k - CK[L], lp = L, fl = 0;
+    T u[3] = {T(0), T(0), T(0)};
+    if (resolve(p.d, yi, yj, yk, lp, fl)) {
+        const int64_t y = ((int64_t)yi * p.d.ny + yj) * p.d.nzp + yk;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T v = (T)p.vel[(int64_t)a * p.nq + y];
+            u[a] = (fl >> a & 1) ? -v : v;
+        }
+    }
+    constexpr int c[3] = {CI[L], CJ[L], CK[L]};
+    constexpr int a = c[0] != 0 ? 0 : (c[1] != 0 ? 1 : 2);
+    constexpr int b = nnz(L) == 1 ? -1 : (a == 0 ? (c[1] != 0 ? 1 : 2) : 2);
+    T term;
+    if constexpr (b < 0) term = cff1 * u[a];
+    else if constexpr (c[a] == c[b]) term = cff2 * (u[a] + u[b]);
+    else term = cff2 * (u[a] - u[b]);
+    if constexpr (c[a] > 0) f[L] = f[L] + term;
+    else f[L] = f[L] - term;
+}
+
+template <class T, int L>
+LBD void ladd_if(const StepParams& p, uint32_t m, T* f, T cff1, T cff2, int i, int j, int k) {
+    if (m & (1u << L)) ladd<T, L>(p, f, cff1, cff2, i, j, k);
+}
+
+template <class T, int... Ls>
+LBD void ladd_all(const StepParams& p, uint32_t m, T* f, T cff1, T cff2, int i, int j, int k,
+                  std::integer_sequence<int, Ls...>) {
+    (ladd_if<T, Ls>(p, m, f, cff1, cff2, i, j, k), ...);
+}
+
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(TILE) void k_step(StepParams p) {
+    using TS = typename Var<P>::TS;
+    using TC = typename Var<P>::TC;
+    constexpr bool SH = Var<P>::SH;
+    const uint32_t tile = xcd_tile();
+    if (tile >= p.ntiles) return;
+    const uint32_t c = tile * TILE + threadIdx.x;
+    if (c >= p.nq) return;
+    const uint32_t m = p.mask[c];
+    if (!(m & M_FLUID)) return;
+
+    TS* F = (TS*)p.f;
+    const int64_t nq = p.nq;
+    const int64_t e = eidx(p, c);
+    int i = 0, j = 0, k = 0;
+    if (m & (M_BND | M_MOVING | M_ZH_ANY)) decompose(c, p.d.ny, p.d.nzp, i, j, k);
+
+    // ---- stream (pull) ----
+    TC f[Q];
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) f[l] = ldf<P>(F + e + (int64_t)l * p.ps, p.inv_scale);
+    } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) f[l] = ldf<P>(F + odd_loc(p, m, e, l, i, j, k), p.inv_scale);
+    }
+    const TC rho0 = (TC)p.rho0;
+
+    // ---- Ladd moving walls on bounce-back links (previous-step density) ----
+    if (p.moving && (m & M_MOVING)) {
+        const TC den = (TC)p.den[c];
+        const TC cff1 = TC(2.0) * den * TC(2.0 / 36.0) * TC(3.0);
+        const TC cff2 = TC(2.0) * den * TC(1.0 / 36.0) * TC(3.0);
+        ladd_all<TC>(p, m, f, cff1, cff2, i, j, k, std::make_integer_sequence<int, Q>{});
+    }
+
+    // ---- Zou-He faces, order 1X, X1, 1Y, Y1, 1Z, Z1 (isothermal.pyx:1280-1291) ----
+    if (m & M_ZH_ANY) {
+#pragma unroll
+        for (int fc = 0; fc < 6; ++fc) {
+            if (!(m & (M_ZH0 << fc))) continue;
+            int64_t fi;
+            if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
+            else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
+            else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
+            TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
+            if (p.d.bc[fc] == FIXED_DEN) zh_fix_den<TC, SH>(fc, (TC)p.fden[fc][fi], vb, rho0, f);
+            else zh_fix_vel<TC, SH>(fc, vb, rho0, f);
+        }
+    }
+
+    // ---- acceleration and collision ----
+    TC acc[3];
+    const TC* A = (const TC*)p.acc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
+    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
+    TC vel[3];
+    const TC den = collide<RLX, TC, SH>(f, acc, prm, rho0, vel);
+
+    if (p.store_dv) {
+        p.den[c] = (double)den;
+        p.vel[c] = (double)vel[0];
+        p.vel[nq + c] = (double)vel[1];
+        p.vel[2 * nq + c] = (double)vel[2];
+    } else if (p.moving && (m & M_MOVING)) {
+        p.den[c] = (double)den;
+    }
+
+    // ---- store ----
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P>(F + e + (int64_t)opp(l) * p.ps, f[l], p.scale);
+    } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P>(F + odd_loc(p, m, e, l, i, j, k), f[opp(l)], p.scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// set-up kernels
+// ---------------------------------------------------------------------------
+
+// Link mask of every site (see lb_lattice.h). `vel` (wall velocities) may be
+// null, then no site is flagged MOVING.
+__global__ void k_mask(const uint8_t* phase, const double* vel, uint32_t* mask, Dims d, uint32_t nq) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    if (k >= d.nz || phase[c] == SOLID) {
+        mask[c] = 0;
+        return;
+    }
+    uint32_t m = M_FLUID;
+    for (int l = 0; l < Q; ++l) {
+        if (l == REST) continue;
+        int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
+        const bool crossed = yi < 0 || yi >= d.nx || yj < 0 || yj >= d.ny || yk < 0 || yk >= d.nz;
+        if (!resolve(d, yi, yj, yk, lp, fl)) {
+            m |= 1u << l;
+            continue;
+        }
+        const int64_t y = ((int64_t)yi * d.ny + yj) * d.nzp + yk;
+        if (phase[y] == SOLID) {
+            m |= 1u << l;
+            if (vel && (vel[y] != 0.0 || vel[nq + y] != 0.0 || vel[2 * (int64_t)nq + y] != 0.0))
+                m |= M_MOVING;
+        } else if (crossed) {
+            m |= M_BND;
+        }
+    }
+    const bool fx[6] = {d.bc[0] == FIXED_DEN || d.bc[0] == FIXED_VEL, d.bc[1] == FIXED_DEN || d.bc[1] == FIXED_VEL,
+                        d.bc[2] == FIXED_DEN || d.bc[2] == FIXED_VEL, d.bc[3] == FIXED_DEN || d.bc[3] == FIXED_VEL,
+                        d.bc[4] == FIXED_DEN || d.bc[4] == FIXED_VEL, d.bc[5] == FIXED_DEN || d.bc[5] == FIXED_VEL};
+    const bool on[6] = {i == 0, i == d.nx - 1, j == 0, j == d.ny - 1, k == 0, k == d.nz - 1};
+    for (int fc = 0; fc < 6; ++fc)
+        if (fx[fc] && on[fc]) m |= M_ZH0 << fc;
+    mask[c] = m;
+}
+
+// velocity of the site reached from x along +/-c_l for _comp_der_vel:
+// resolved through periodic / symmetric halos, zero for WALL/FIXED halos
+LBD void nb_vel(const Dims& d, const double* vel, uint32_t nq, int i, int j, int k, double* u) {
+    int l = 0, fl = 0;
+    u[0] = u[1] = u[2] = 0.0;
+    if (!resolve(d, i, j, k, l, fl)) return;
+    const int64_t y = ((int64_t)i * d.ny + j) * d.nzp + k;
+    for (int a = 0; a < 3; ++a) {
+        double v = vel[(int64_t)a * nq + y];
+        u[a] = (fl >> a & 1) ? -v : v;
+    }
+}
+
+// Solver._comp_der_vel, isothermal.pyx:958-1023
+__device__ void comp_der_vel(const Dims& d, const double* vel, uint32_t nq, uint32_t m, int i, int j, int k,
+                             int fdir, const double* u0, double* der) {
+    constexpr int FD[3][5] = {{14, 15, 16, 17, 18}, {4, 11, 12, 13, 18}, {3, 7, 10, 13, 17}};
+    der[0] = der[1] = der[2] = 0.0;
+    for (int fl = 0; fl < 5; ++fl) {
+        const int l = FD[fdir][fl];
+        double un[3], uo[3];
+        nb_vel(d, vel, nq, i + CI[l], j + CJ[l], k + CK[l], un);
+        nb_vel(d, vel, nq, i - CI[l], j - CJ[l], k - CK[l], uo);
+        const bool sn = m & (1u << opp(l));  // x + c_l is SOLID
+        const bool so = m & (1u << l);       // x - c_l is SOLID
+        double c1 = 0.5, c2 = 0.0, c3 = -0.5;
+        if (sn) {
+            if (so) {
+                c1 = 1.0;
+                c3 = -1.0;
+            } else {
+                c1 = 4.0 / 3.0;
+                c2 = -1.0;
+                c3 = -1.0 / 3.0;
+            }
+        } else if (so) {
+            c1 = 1.0 / 3.0;
+            c2 = 1.0;
+            c3 = -4.0 / 3.0;
+        }
+        const double w = nnz(l) == 1 ? 2.0 / 36.0 : 1.0 / 36.0;
+        const double c0 = 2.0 * w * 3.0;
+        der[0] += c0 * (c1 * un[0] + c2 * u0[0] + c3 * uo[0]);
+        der[1] += c0 * (c1 * un[1] + c2 * u0[1] + c3 * uo[1]);
+        der[2] += c0 * (c1 * un[2] + c2 * u0[2] + c3 * uo[2]);
+    }
+}
+
+// Solver._init_f, isothermal.pyx:1028-1105: f1 = P(heq + (1-omega)(-2 tau/3) rho S),
+// written as the state after an even AA step (f*_l in slot opp(l)).
+template <int P>
+__global__ void k_init(StepParams p) {
+    using TS = typename Var<P>::TS;
+    constexpr bool SH = Var<P>::SH;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nq) return;
+    const uint32_t m = p.mask[c];
+    if (!(m & M_FLUID)) return;
+    int i, j, k;
+    decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    const int64_t nq = p.nq;
+    const double inv_tau = p.prm[0], tau = 1.0 / inv_tau, rlx_cff = 1.0 - inv_tau;
+    const double den = p.den[c];
+    double u[3] = {p.vel[c], p.vel[nq + c], p.vel[2 * nq + c]};
+    double h[10], dx[3], dy[3], dz[3], f[Q];
+    heq<double>(den, u, h);
+    comp_der_vel(p.d, p.vel, p.nq, m, i, j, k, 0, u, dx);
+    comp_der_vel(p.d, p.vel, p.nq, m, i, j, k, 1, u, dy);
+    comp_der_vel(p.d, p.vel, p.nq, m, i, j, k, 2, u, dz);
+    const double m2 = -2.0 * tau * (1.0 / 3.0) * den;
+    h[4] += rlx_cff * m2 * dx[0];
+    h[5] += rlx_cff * m2 * 0.5 * (dx[1] + dy[0]);
+    h[6] += rlx_cff * m2 * 0.5 * (dx[2] + dz[0]);
+    h[7] += rlx_cff * m2 * dy[1];
+    h[8] += rlx_cff * m2 * 0.5 * (dy[2] + dz[1]);
+    h[9] += rlx_cff * m2 * dz[2];
+    if (SH) h[0] = den - p.rho0;
+    project<double, false>(h, f);
+    TS* F = (TS*)p.f;
+    const int64_t e = eidx(p, c);
+    for (int l = 0; l < Q; ++l) stf<P>(F + e + (int64_t)opp(l) * p.ps, (typename Var<P>::TC)f[l], p.scale);
+}
+
+// Post-collision populations of the newest step, unpacked into the reference's
+// f1 layout restricted to the interior: out[l][i][j][k] (f64).
+template <int P>
+__global__ void k_post_f(StepParams p, int next_odd, double* out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nq) return;
+    int i, j, k;
+    decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    if (k >= p.d.nz) return;
+    const int64_t nc = (int64_t)p.d.nx * p.d.ny * p.d.nz;
+    const int64_t o = ((int64_t)i * p.d.ny + j) * p.d.nz + k;
+    const uint32_t m = p.mask[c];
+    const typename Var<P>::TS* F = (const typename Var<P>::TS*)p.f;
+    for (int l = 0; l < Q; ++l) {
+        double v = 0.0;
+        if (m & M_FLUID) {
+            const int64_t e = eidx(p, c);
+            const int64_t loc = next_odd ? e + (int64_t)opp(l) * p.ps : odd_loc(p, m, e, opp(l), i, j, k);
+            v = (double)ldf<P>(F + loc, p.inv_scale);
+            if (Var<P>::SH) v += (l == REST ? 12.0 / 36.0 : nnz(l) == 1 ? 2.0 / 36.0 : 1.0 / 36.0) * p.rho0;
+        }
+        out[(int64_t)l * nc + o] = v;
+    }
+}
+
+// interior contiguous (den[nc], vel[nc][3]) <-> padded SoA device fields
+__global__ void k_import_dv(const double* st, double* den, double* vel, Dims d, uint32_t nq, int with_den) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    if (k >= d.nz) return;
+    const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t o = ((int64_t)i * d.ny + j) * d.nz + k;
+    if (with_den) den[c] = st[o];
+    const double* v = st + (with_den ? nc : 0) + 3 * o;
+    vel[c] = v[0];
+    vel[nq + c] = v[1];
+    vel[2 * (int64_t)nq + c] = v[2];
+}
+
+__global__ void k_export_dv(double* st, const double* den, const double* vel, Dims d, uint32_t nq) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    if (k >= d.nz) return;
+    const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t o = ((int64_t)i * d.ny + j) * d.nz + k;
+    st[o] = den[c];
+    double* v = st + nc + 3 * o;
+    v[0] = vel[c];
+    v[1] = vel[nq + c];
+    v[2] = vel[2 * (int64_t)nq + c];
+}
+
+template <class TC>
+__global__ void k_import_acc(const double* st, TC* acc, Dims d, uint32_t nq) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    if (k >= d.nz) {
+        acc[c] = acc[nq + c] = acc[2 * (int64_t)nq + c] = TC(0);
+        return;
+    }
+    const int64_t o = ((int64_t)i * d.ny + j) * d.nz + k;
+    acc[c] = (TC)st[3 * o];
+    acc[nq + c] = (TC)st[3 * o + 1];
+    acc[2 * (int64_t)nq + c] = (TC)st[3 * o + 2];
+}
+
+__global__ void k_import_phase(const uint8_t* st, uint8_t* phase, Dims d, uint32_t nq) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    phase[c] = k < d.nz ? st[((int64_t)i * d.ny + j) * d.nz + k] : (uint8_t)SOLID;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions: wave shuffles -> LDS -> per-block partials -> one block
+// ---------------------------------------------------------------------------
+constexpr int RED_N = 8;
+constexpr int RED_BLOCK = 256;
+
+LBD double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+LBD double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_reduce1(const uint32_t* mask, const double* den, const double* vel,
+                                                      uint32_t nq, double* part) {
+    double a[RED_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t c = blockIdx.x * RED_BLOCK + threadIdx.x; c < nq; c += gridDim.x * RED_BLOCK) {
+        if (!(mask[c] & M_FLUID)) continue;
+        const double r = den[c], u = vel[c], v = vel[nq + c], w = vel[2 * (int64_t)nq + c];
+        a[0] += r;
+        a[1] += u;
+        a[2] += v;
+        a[3] += w;
+        a[4] += r * u;
+        a[5] += r * v;
+        a[6] += r * w;
+        a[7] = fmax(a[7], sqrt(u * u + v * v + w * w));
+    }
+    __shared__ double sh[RED_BLOCK / 64][RED_N];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < RED_N; ++q) {
+        double s = q == 7 ? wave_max(a[q]) : wave_sum(a[q]);
+        if (lane == 0) sh[wv][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < RED_N) {
+        const int q = threadIdx.x;
+        double s = sh[0][q];
+        for (int w2 = 1; w2 < RED_BLOCK / 64; ++w2) s = q == 7 ? fmax(s, sh[w2][q]) : s + sh[w2][q];
+        part[(int64_t)blockIdx.x * RED_N + q] = s;
+    }
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_reduce2(const double* part, int nparts, double* out) {
+    __shared__ double sh[RED_BLOCK / 64][RED_N];
+    double a[RED_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nparts; b += RED_BLOCK)
+        for (int q = 0; q < RED_N; ++q) a[q] = q == 7 ? fmax(a[q], part[(int64_t)b * RED_N + q]) : a[q] + part[(int64_t)b * RED_N + q];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < RED_N; ++q) {
+        double s = q == 7 ? wave_max(a[q]) : wave_sum(a[q]);
+        if (lane == 0) sh[wv][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < RED_N) {
+        const int q = threadIdx.x;
+        double s = sh[0][q];
+        for (int w2 = 1; w2 < RED_BLOCK / 64; ++w2) s = q == 7 ? fmax(s, sh[w2][q]) : s + sh[w2][q];
+        out[q] = s;
+    }
+}
+
+// sum |u - u_prev|^2 and sum |u|^2 over fluid sites, then u_prev := u
+__global__ __launch_bounds__(RED_BLOCK) void k_conv1(const uint32_t* mask, const double* vel, double* uprev,
+                                                    uint32_t nq, double* part) {
+    double a[RED_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t c = blockIdx.x * RED_BLOCK + threadIdx.x; c < nq; c += gridDim.x * RED_BLOCK) {
+        if (!(mask[c] & M_FLUID)) continue;
+        for (int q = 0; q < 3; ++q) {
+            const double u = vel[(int64_t)q * nq + c], p = uprev[(int64_t)q * nq + c];
+            a[0] += (u - p) * (u - p);
+            a[1] += u * u;
+            uprev[(int64_t)q * nq + c] = u;
+        }
+    }
+    __shared__ double sh[RED_BLOCK / 64][RED_N];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < RED_N; ++q) {
+        double s = wave_sum(a[q]);
+        if (lane == 0) sh[wv][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < RED_N) {
+        const int q = threadIdx.x;
+        double s = sh[0][q];
+        for (int w2 = 1; w2 < RED_BLOCK / 64; ++w2) s += sh[w2][q];
+        part[(int64_t)blockIdx.x * RED_N + q] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct lb_solver {
+    lb_config cfg;
+    Dims d;
+    int NI, NJ, NK;
+    uint32_t nq;
+    int64_t ncell;
+    int64_t rs, ps, fsize;  // population-array strides and element count
+    int prec;
+    double prm[2];
+    double rho0;
+    double scale;
+    hipStream_t stream = nullptr;
+    void* f = nullptr;
+    uint32_t* mask = nullptr;
+    uint8_t* phase = nullptr;
+    double* den = nullptr;
+    double* vel = nullptr;
+    void* acc = nullptr;
+    double* fden[6] = {};
+    double* fvel[6] = {};
+    double* stage = nullptr;  // 4 * ncell doubles (import / export)
+    double* part = nullptr;   // reduction partials
+    double* red = nullptr;    // 8 doubles
+    double* uprev = nullptr;
+    bool have_uprev = false;
+    bool phase_set = false;
+    bool finalized = false;
+    bool moving = false;
+    int64_t nfluid = 0;
+    uint64_t steps = 0;
+    int next_odd = 1;
+    uint64_t bytes = 0;
+};
+
+static constexpr int RED_PARTS = 1024;
+
+static size_t ts_size(int prec) { return prec == LB_FP64 ? 8 : prec == LB_FP32_SHIFTED ? 4 : 2; }
+static size_t tc_size(int prec) { return prec == LB_FP64 ? 8 : 4; }
+
+static int dalloc(lb_solver* s, void** p, size_t n) {
+    hipError_t e = hipMalloc(p, n);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LB_ENOMEM, "hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(e));
+    }
+    s->bytes += n;
+    return LB_OK;
+}
+
+static StepParams params(const lb_solver* s) {
+    StepParams p;
+    memset(&p, 0, sizeof(p));
+    p.f = s->f;
+    p.mask = s->mask;
+    p.den = s->den;
+    p.vel = s->vel;
+    p.acc = s->acc;
+    for (int fc = 0; fc < 6; ++fc) {
+        p.fden[fc] = s->fden[fc];
+        p.fvel[fc] = s->fvel[fc];
+    }
+    for (int a = 0; a < 3; ++a) p.g[a] = s->cfg.g[a];
+    p.prm[0] = s->prm[0];
+    p.prm[1] = s->prm[1];
+    p.rho0 = s->rho0;
+    p.scale = (float)s->scale;
+    p.inv_scale = (float)(1.0 / s->scale);
+    p.d = s->d;
+    p.NJ = s->NJ;
+    p.NK = s->NK;
+    p.nq = s->nq;
+    p.ntiles = (s->nq + TILE - 1) / TILE;
+    p.rs = s->rs;
+    p.ps = s->ps;
+    p.moving = s->moving ? 1 : 0;
+    return p;
+}
+
+static unsigned blocks_for(uint32_t n, unsigned b) { return (n + b - 1) / b; }
+
+// _check_bc, isothermal.pyx:378-426
+static int check_config(const lb_config* c) {
+    if (c->lat[0] < 1 || c->lat[1] < 1 || c->lat[2] < 1) return fail(LB_EINVAL, "Invalid lattice size input.");
+    if (c->rlx_scheme < 0 || c->rlx_scheme > 2) return fail(LB_EINVAL, "Invalid relaxation scheme input.");
+    if (!(c->kvisc > 0.0)) return fail(LB_EINVAL, "Invalid kinematic viscosity input.");
+    for (int f = 0; f < 6; ++f)
+        if (c->bc[f] < WALL || c->bc[f] > SYMMETRIC) return fail(LB_EINVAL, "Invalid boundary condition input.");
+    const char* ax = "xyz";
+    for (int a = 0; a < 3; ++a) {
+        const int lo = c->bc[2 * a], hi = c->bc[2 * a + 1];
+        if ((lo == PERIODIC || hi == PERIODIC) && lo != hi)
+            return fail(LB_EINVAL, "Invalid periodic bc setup (in %c-dir.).", ax[a]);
+    }
+    bool fx[3];
+    for (int a = 0; a < 3; ++a) {
+        const int lo = c->bc[2 * a], hi = c->bc[2 * a + 1];
+        fx[a] = lo == FIXED_DEN || lo == FIXED_VEL || hi == FIXED_DEN || hi == FIXED_VEL;
+    }
+    if ((fx[0] && (fx[1] || fx[2])) || (fx[1] && (fx[0] || fx[2])) || (fx[2] && (fx[0] || fx[1])))
+        return fail(LB_EINVAL, "Invalid fixed variable bc setup.");
+    if (c->precision < LB_FP64 || c->precision > LB_FP16_STORAGE) return fail(LB_EINVAL, "Invalid precision.");
+    if (c->layout < LB_LAYOUT_ROWS || c->layout > LB_LAYOUT_SOA_PADDED) return fail(LB_EINVAL, "Invalid layout.");
+    const int64_t nzp = (c->lat[2] + 63) / 64 * 64;
+    const int64_t nq = (int64_t)c->lat[0] * c->lat[1] * nzp;
+    if (nq >= (int64_t)1 << 31) return fail(LB_EINVAL, "Lattice too large (%lld padded sites).", (long long)nq);
+    return LB_OK;
+}
+
+static int ensure_stage(lb_solver* s) {
+    if (s->stage) return LB_OK;
+    return dalloc(s, (void**)&s->stage, (size_t)4 * s->ncell * 8);
+}
+
+static int import_acc_from_stage(lb_solver* s) {
+    if (!s->acc) {
+        int rc = dalloc(s, &s->acc, (size_t)3 * s->nq * tc_size(s->prec));
+        if (rc) return rc;
+    }
+    if (s->prec == LB_FP64)
+        k_import_acc<double><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, (double*)s->acc, s->d, s->nq);
+    else
+        k_import_acc<float><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, (float*)s->acc, s->d, s->nq);
+    HIPCHK(hipGetLastError());
+    return LB_OK;
+}
+
+static int build_mask(lb_solver* s, bool with_walls) {
+    k_mask<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->phase, with_walls ? s->vel : nullptr, s->mask, s->d,
+                                                          s->nq);
+    HIPCHK(hipGetLastError());
+    return LB_OK;
+}
+
+// count fluid sites and moving-wall sites from the mask
+__global__ void k_count(const uint32_t* mask, uint32_t nq, unsigned long long* out) {
+    __shared__ unsigned long long sh[2];
+    if (threadIdx.x < 2) sh[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long a = 0, b = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nq; c += gridDim.x * blockDim.x) {
+        const uint32_t m = mask[c];
+        a += (m & M_FLUID) ? 1 : 0;
+        b += (m & M_MOVING) ? 1 : 0;
+    }
+    atomicAdd(&sh[0], a);
+    atomicAdd(&sh[1], b);
+    __syncthreads();
+    if (threadIdx.x < 2) atomicAdd(&out[threadIdx.x], sh[threadIdx.x]);
+}
+
+static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
+    unsigned long long* d = (unsigned long long*)s->red;
+    HIPCHK(hipMemsetAsync(d, 0, 16, s->stream));
+    k_count<<<256, 256, 0, s->stream>>>(s->mask, s->nq, d);
+    HIPCHK(hipGetLastError());
+    unsigned long long h[2];
+    HIPCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    *nfluid = (int64_t)h[0];
+    *nmoving = (int64_t)h[1];
+    return LB_OK;
+}
+
+// max |f - w rho0| of the initial state, for the FP16 storage scale
+__global__ void k_maxdev(const uint32_t* mask, const double* den, const double* vel, uint32_t nq, double rho0,
+                         double* part) {
+    double a = 0.0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nq; c += gridDim.x * blockDim.x) {
+        const double u = fabs(vel[c]) + fabs(vel[nq + c]) + fabs(vel[2 * (int64_t)nq + c]);
+        if (mask[c] & M_FLUID) a = fmax(a, fabs(den[c] - rho0) + 3.0 * u + 4.5 * u * u);
+        else a = fmax(a, 3.0 * u + 4.5 * u * u);  // wall velocities
+    }
+    a = wave_max(a);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+template <int P>
+static void launch_init(lb_solver* s) {
+    StepParams p = params(s);
+    k_init<P><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p);
+}
+
+template <int P, int RLX>
+static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
+    const unsigned grid = (p.ntiles + 7) / 8 * 8;
+    if (odd) k_step<P, RLX, true><<<grid, TILE, 0, st>>>(p);
+    else k_step<P, RLX, false><<<grid, TILE, 0, st>>>(p);
+}
+
+template <int P>
+static void launch_step_rlx(const StepParams& p, int rlx, bool odd, hipStream_t st) {
+    if (rlx == BGK) launch_step<P, BGK>(p, odd, st);
+    else if (rlx == TRT) launch_step<P, TRT>(p, odd, st);
+    else launch_step<P, REGUL>(p, odd, st);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* lb_last_error(void) { return g_err.c_str(); }
+
+int lb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int lb_create(const lb_config* cfg, lb_solver** out) {
+    if (!cfg || !out) return fail(LB_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    if (lb_device_count() < 1) return fail(LB_EHIP, "no HIP device available");
+    lb_solver* s = new lb_solver();
+    s->cfg = *cfg;
+    s->d.nx = cfg->lat[0];
+    s->d.ny = cfg->lat[1];
+    s->d.nz = cfg->lat[2];
+    s->d.nzp = (cfg->lat[2] + 63) / 64 * 64;
+    for (int f = 0; f < 6; ++f) s->d.bc[f] = cfg->bc[f];
+    s->NI = s->d.nx + 2;
+    s->NJ = s->d.ny + 2;
+    s->NK = s->d.nz + 2;
+    s->nq = (uint32_t)((int64_t)s->d.nx * s->d.ny * s->d.nzp);
+    s->ncell = (int64_t)s->d.nx * s->d.ny * s->d.nz;
+    s->prec = cfg->precision;
+    s->rho0 = cfg->rho0 > 0 ? cfg->rho0 : 1.0;
+    s->scale = cfg->fp16_scale > 0 ? cfg->fp16_scale : 1.0;
+    // relaxation parameters, isothermal.pyx:273-287
+    const double tau_e = 0.5 + 3.0 * cfg->kvisc;
+    const double inv_tau_e = 1.0 / tau_e;
+    s->prm[0] = inv_tau_e;
+    s->prm[1] = cfg->rlx_scheme == TRT ? 8.0 * (2.0 - inv_tau_e) / (8.0 - inv_tau_e) : 0.0;
+    int8_t mir[3][Q];
+    for (int a = 0; a < 3; ++a)
+        for (int l = 0; l < Q; ++l) mir[a][l] = (int8_t)mirror(l, a);
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_mirror), mir, sizeof(mir)) != hipSuccess) {
+        delete s;
+        return fail(LB_EHIP, "hipMemcpyToSymbol failed");
+    }
+    const size_t nq = s->nq;
+    // population layout: row-interleaved f[i][j][l][k] (default), plain SoA
+    // f[l][i][j][k], or SoA with the population stride padded by 2304 bytes
+    // so that equal offsets of different populations do not share a channel
+    const int64_t rows = (int64_t)s->d.nx * s->d.ny;
+    if (cfg->layout == LB_LAYOUT_SOA || cfg->layout == LB_LAYOUT_SOA_PADDED) {
+        s->rs = s->d.nzp;
+        s->ps = (int64_t)nq + (cfg->layout == LB_LAYOUT_SOA_PADDED ? (int64_t)(2304 / ts_size(s->prec)) : 0);
+    } else {
+        s->rs = (int64_t)Q * s->d.nzp;
+        s->ps = s->d.nzp;
+    }
+    s->fsize = cfg->layout == LB_LAYOUT_SOA || cfg->layout == LB_LAYOUT_SOA_PADDED ? (int64_t)Q * s->ps
+                                                                                  : rows * s->rs;
+    if ((rc = dalloc(s, &s->f, (size_t)s->fsize * ts_size(s->prec))) ||
+        (rc = dalloc(s, (void**)&s->mask, nq * 4)) || (rc = dalloc(s, (void**)&s->phase, nq)) ||
+        (rc = dalloc(s, (void**)&s->den, nq * 8)) || (rc = dalloc(s, (void**)&s->vel, 3 * nq * 8)) ||
+        (rc = dalloc(s, (void**)&s->part, (size_t)RED_PARTS * RED_N * 8)) ||
+        (rc = dalloc(s, (void**)&s->red, RED_N * 8))) {
+        std::string msg = g_err;
+        lb_destroy(s);
+        g_err = msg;
+        return rc;
+    }
+    // den = 1, vel = 0, phase SOLID, mask 0 (isothermal.pyx:243-248)
+    (void)hipMemset(s->f, 0, (size_t)s->fsize * ts_size(s->prec));
+    (void)hipMemset(s->mask, 0, nq * 4);
+    (void)hipMemset(s->phase, 0, nq);
+    (void)hipMemset(s->vel, 0, 3 * nq * 8);
+    std::vector<double> ones;  // den = 1 via a host copy of one row block
+    {
+        const size_t chunk = 1 << 20;
+        ones.assign(chunk, 1.0);
+        for (size_t o = 0; o < nq; o += chunk)
+            (void)hipMemcpy(s->den + o, ones.data(), std::min(chunk, nq - o) * 8, hipMemcpyHostToDevice);
+    }
+    for (int f = 0; f < 6; ++f) {
+        if (cfg->bc[f] != FIXED_DEN && cfg->bc[f] != FIXED_VEL) continue;
+        const size_t n = (size_t)(f < 2 ? s->NJ * s->NK : f < 4 ? s->NI * s->NK : s->NI * s->NJ);
+        if ((rc = dalloc(s, (void**)&s->fden[f], n * 8)) || (rc = dalloc(s, (void**)&s->fvel[f], 3 * n * 8))) {
+            std::string msg = g_err;
+            lb_destroy(s);
+            g_err = msg;
+            return rc;
+        }
+        std::vector<double> o1(n, 1.0), z3(3 * n, 0.0);
+        (void)hipMemcpy(s->fden[f], o1.data(), n * 8, hipMemcpyHostToDevice);
+        (void)hipMemcpy(s->fvel[f], z3.data(), 3 * n * 8, hipMemcpyHostToDevice);
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        lb_destroy(s);
+        return fail(LB_EHIP, "device initialisation failed");
+    }
+    *out = s;
+    return LB_OK;
+}
+
+int lb_destroy(lb_solver* s) {
+    if (!s) return LB_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    else (void)hipDeviceSynchronize();
+    void* ptrs[] = {s->f, s->mask, s->phase, s->den, s->vel, s->acc, s->stage, s->part, s->red, s->uprev};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int f = 0; f < 6; ++f) {
+        if (s->fden[f]) (void)hipFree(s->fden[f]);
+        if (s->fvel[f]) (void)hipFree(s->fvel[f]);
+    }
+    delete s;
+    return LB_OK;
+}
+
+int lb_set_stream(lb_solver* s, void* st) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    s->stream = (hipStream_t)st;
+    return LB_OK;
+}
+
+int lb_set_gravity(lb_solver* s, const double g[3]) {
+    if (!s || !g) return fail(LB_EINVAL, "null argument");
+    for (int a = 0; a < 3; ++a) s->cfg.g[a] = g[a];
+    return LB_OK;
+}
+
+int lb_upload_phase(lb_solver* s, const uint8_t* phase) {
+    if (!s || !phase) return fail(LB_EINVAL, "null argument");
+    int rc = ensure_stage(s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->stage, phase, (size_t)s->ncell, hipMemcpyHostToDevice, s->stream));
+    k_import_phase<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>((const uint8_t*)s->stage, s->phase, s->d, s->nq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->phase_set = true;
+    return LB_OK;
+}
+
+int lb_upload_den_vel(lb_solver* s, const double* den, const double* vel) {
+    if (!s || !den || !vel) return fail(LB_EINVAL, "null argument");
+    int rc = ensure_stage(s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->stage, den, (size_t)s->ncell * 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->stage + s->ncell, vel, (size_t)s->ncell * 24, hipMemcpyHostToDevice, s->stream));
+    k_import_dv<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, s->den, s->vel, s->d, s->nq, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_upload_acc(lb_solver* s, const double* acc) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!acc) {
+        if (s->acc) {
+            HIPCHK(hipStreamSynchronize(s->stream));
+            (void)hipFree(s->acc);
+            s->bytes -= (size_t)3 * s->nq * tc_size(s->prec);
+            s->acc = nullptr;
+        }
+        return LB_OK;
+    }
+    int rc = ensure_stage(s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->stage, acc, (size_t)s->ncell * 24, hipMemcpyHostToDevice, s->stream));
+    if ((rc = import_acc_from_stage(s))) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_upload_acc_device(lb_solver* s, const double* acc_dev) {
+    if (!s || !acc_dev) return fail(LB_EINVAL, "null argument");
+    int rc = ensure_stage(s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->stage, acc_dev, (size_t)s->ncell * 24, hipMemcpyDeviceToDevice, s->stream));
+    return import_acc_from_stage(s);
+}
+
+int lb_set_face_data(lb_solver* s, int face, const double* den_face, const double* vel_face) {
+    if (!s || face < 0 || face > 5 || !den_face || !vel_face) return fail(LB_EINVAL, "invalid argument");
+    if (!s->fden[face]) return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", face);
+    const size_t n = (size_t)(face < 2 ? s->NJ * s->NK : face < 4 ? s->NI * s->NK : s->NI * s->NJ);
+    HIPCHK(hipMemcpyAsync(s->fden[face], den_face, n * 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->fvel[face], vel_face, 3 * n * 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_finalize(lb_solver* s, int64_t* nfluid) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
+    int rc = build_mask(s, true);
+    if (rc) return rc;
+    int64_t nf, nm;
+    if ((rc = count_mask(s, &nf, &nm))) return rc;
+    s->nfluid = nf;
+    s->moving = nm > 0;
+    if (s->prec == LB_FP16_STORAGE && !(s->cfg.fp16_scale > 0)) {
+        k_maxdev<<<256, 256, 0, s->stream>>>(s->mask, s->den, s->vel, s->nq, s->rho0, s->part);
+        HIPCHK(hipGetLastError());
+        std::vector<double> h(256);
+        HIPCHK(hipMemcpyAsync(h.data(), s->part, 256 * 8, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        double mx = 0.0;
+        for (double v : h) mx = fmax(mx, v);
+        // populations are O(w * mx) with w <= 1/3; keep them near 2^10 so that the
+        // f16 range (6.1e-5 .. 65504) leaves ~64x headroom for growth
+        const double est = fmax(mx / 3.0, 1e-4);
+        s->scale = ldexp(1.0, (int)floor(log2(1024.0 / est)));
+        s->scale = fmin(fmax(s->scale, 1.0), ldexp(1.0, 24));
+    }
+    switch (s->prec) {
+    case LB_FP64: launch_init<LB_FP64>(s); break;
+    case LB_FP32_SHIFTED: launch_init<LB_FP32_SHIFTED>(s); break;
+    default: launch_init<LB_FP16_STORAGE>(s); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->finalized = true;
+    s->steps = 0;
+    s->next_odd = 1;
+    s->have_uprev = false;
+    if (nfluid) *nfluid = nf;
+    return LB_OK;
+}
+
+int lb_refresh_walls(lb_solver* s) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
+    int rc = build_mask(s, true);
+    if (rc) return rc;
+    int64_t nf, nm;
+    if ((rc = count_mask(s, &nf, &nm))) return rc;
+    s->moving = nm > 0;
+    return LB_OK;
+}
+
+int lb_evolve(lb_solver* s, uint32_t steps) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->finalized) return fail(LB_ESTATE, "Finalize solver initialization before calling evolve");
+    if (s->nfluid < 1 || steps == 0) return LB_OK;
+    StepParams p = params(s);
+    for (uint32_t t = 0; t < steps; ++t) {
+        p.store_dv = (t + 1 == steps) ? 1 : 0;
+        const bool odd = s->next_odd != 0;
+        switch (s->prec) {
+        case LB_FP64: launch_step_rlx<LB_FP64>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+        case LB_FP32_SHIFTED: launch_step_rlx<LB_FP32_SHIFTED>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+        default: launch_step_rlx<LB_FP16_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+        }
+        s->next_odd ^= 1;
+        s->steps++;
+    }
+    HIPCHK(hipGetLastError());
+    return LB_OK;
+}
+
+int lb_download_den_vel(lb_solver* s, double* den, double* vel) {
+    if (!s || !den || !vel) return fail(LB_EINVAL, "null argument");
+    int rc = ensure_stage(s);
+    if (rc) return rc;
+    k_export_dv<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, s->den, s->vel, s->d, s->nq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(den, s->stage, (size_t)s->ncell * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(vel, s->stage + s->ncell, (size_t)s->ncell * 24, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_download_den_vel_device(lb_solver* s, double* den_dev, double* vel_dev) {
+    if (!s || !den_dev || !vel_dev) return fail(LB_EINVAL, "null argument");
+    int rc = ensure_stage(s);
+    if (rc) return rc;
+    k_export_dv<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, s->den, s->vel, s->d, s->nq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(den_dev, s->stage, (size_t)s->ncell * 8, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(vel_dev, s->stage + s->ncell, (size_t)s->ncell * 24, hipMemcpyDeviceToDevice, s->stream));
+    return LB_OK;
+}
+
+int lb_download_post_f(lb_solver* s, double* f) {
+    if (!s || !f) return fail(LB_EINVAL, "null argument");
+    if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
+    double* buf = nullptr;
+    int rc = dalloc(s, (void**)&buf, (size_t)Q * s->ncell * 8);
+    if (rc) return rc;
+    StepParams p = params(s);
+    switch (s->prec) {
+    case LB_FP64: k_post_f<LB_FP64><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf); break;
+    case LB_FP32_SHIFTED:
+        k_post_f<LB_FP32_SHIFTED><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
+        break;
+    default:
+        k_post_f<LB_FP16_STORAGE><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
+        break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(f, buf, (size_t)Q * s->ncell * 8, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(buf);
+    s->bytes -= (size_t)Q * s->ncell * 8;
+    if (e != hipSuccess) return fail(LB_EHIP, "lb_download_post_f: %s", hipGetErrorString(e));
+    return LB_OK;
+}
+
+int lb_reduce(lb_solver* s, double out[8]) {
+    if (!s || !out) return fail(LB_EINVAL, "null argument");
+    const int nb = (int)std::min<uint32_t>(RED_PARTS, blocks_for(s->nq, RED_BLOCK));
+    k_reduce1<<<nb, RED_BLOCK, 0, s->stream>>>(s->mask, s->den, s->vel, s->nq, s->part);
+    k_reduce2<<<1, RED_BLOCK, 0, s->stream>>>(s->part, nb, s->red);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, s->red, RED_N * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_convergence(lb_solver* s, double* rel) {
+    if (!s || !rel) return fail(LB_EINVAL, "null argument");
+    if (!s->uprev) {
+        int rc = dalloc(s, (void**)&s->uprev, (size_t)3 * s->nq * 8);
+        if (rc) return rc;
+    }
+    const int nb = (int)std::min<uint32_t>(RED_PARTS, blocks_for(s->nq, RED_BLOCK));
+    k_conv1<<<nb, RED_BLOCK, 0, s->stream>>>(s->mask, s->vel, s->uprev, s->nq, s->part);
+    k_reduce2<<<1, RED_BLOCK, 0, s->stream>>>(s->part, nb, s->red);
+    HIPCHK(hipGetLastError());
+    double h[RED_N];
+    HIPCHK(hipMemcpyAsync(h, s->red, RED_N * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (!s->have_uprev) {
+        s->have_uprev = true;
+        *rel = 1.0;
+    } else {
+        *rel = h[1] > 0 ? sqrt(h[0] / h[1]) : (h[0] > 0 ? INFINITY : 0.0);
+    }
+    return LB_OK;
+}
+
+int lb_info(lb_solver* s, uint64_t* steps, int* next_odd, uint64_t* bytes, double* scale) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (steps) *steps = s->steps;
+    if (next_odd) *next_odd = s->next_odd;
+    if (bytes) *bytes = s->bytes;
+    if (scale) *scale = s->scale;
+    return LB_OK;
+}
+
+int lb_synchronize(lb_solver* s) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+}  // extern "C"

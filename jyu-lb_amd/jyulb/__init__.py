@@ -1,0 +1,11 @@
+"""jyulb: MI355X-native drop-in for the evolve path of the JYU-LB isothermal
+D3Q19 lattice-Boltzmann solver.
+
+Same module layout and public names as the reference package
+(jyulb/__init__.py:1): isothermal (Solver), D3Q19 (velocity set and moment
+constants), bcs (Zou-He face conventions), defs (enumerations) and
+flow_field (analytic profiles). Solver.evolve runs hand-written HIP kernels
+for gfx950 through the C ABI of libjyulb_hip.so (include/jyulb_gpu.h).
+"""
+__all__ = ['isothermal', 'D3Q19', 'bcs', 'defs', 'flow_field']
+__version__ = '0.2.0'

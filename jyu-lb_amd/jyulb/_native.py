@@ -1,0 +1,201 @@
+"""ctypes binding of libjyulb_hip.so (C ABI: include/jyulb_gpu.h).
+
+The library is built in-tree by jyu-lb_amd/build.py. There is no fallback:
+if the library or a HIP device is missing, using the solver raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libjyulb_hip.so")
+
+LB_OK, LB_EINVAL, LB_ESTATE, LB_EHIP, LB_ENOMEM = 0, 1, 2, 3, 4
+LB_FP64, LB_FP32_SHIFTED, LB_FP16_STORAGE = 0, 1, 2
+
+PRECISIONS = {"fp64": LB_FP64, "fp32": LB_FP32_SHIFTED, "fp32_shifted": LB_FP32_SHIFTED,
+              "fp16": LB_FP16_STORAGE, "fp16_storage": LB_FP16_STORAGE,
+              LB_FP64: LB_FP64, LB_FP32_SHIFTED: LB_FP32_SHIFTED, LB_FP16_STORAGE: LB_FP16_STORAGE}
+
+
+class LbConfig(ctypes.Structure):
+    _fields_ = [
+        ("lat", ctypes.c_int32 * 3),
+        ("rlx_scheme", ctypes.c_int32),
+        ("kvisc", ctypes.c_double),
+        ("g", ctypes.c_double * 3),
+        ("bc", ctypes.c_int32 * 6),
+        ("precision", ctypes.c_int32),
+        ("rho0", ctypes.c_double),
+        ("fp16_scale", ctypes.c_double),
+        ("layout", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the engine; raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not found: build it with `python jyu-lb_amd/build.py`")
+    L = ctypes.CDLL(LIB_PATH)
+    H = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "lb_last_error": ([], ctypes.c_char_p),
+        "lb_device_count": ([], ctypes.c_int),
+        "lb_create": ([ctypes.POINTER(LbConfig), ctypes.POINTER(H)], ctypes.c_int),
+        "lb_destroy": ([H], ctypes.c_int),
+        "lb_set_stream": ([H, ctypes.c_void_p], ctypes.c_int),
+        "lb_set_gravity": ([H, dp], ctypes.c_int),
+        "lb_upload_phase": ([H, ctypes.c_void_p], ctypes.c_int),
+        "lb_upload_den_vel": ([H, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "lb_upload_acc": ([H, ctypes.c_void_p], ctypes.c_int),
+        "lb_set_face_data": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "lb_finalize": ([H, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "lb_evolve": ([H, ctypes.c_uint32], ctypes.c_int),
+        "lb_refresh_walls": ([H], ctypes.c_int),
+        "lb_download_den_vel": ([H, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "lb_download_post_f": ([H, ctypes.c_void_p], ctypes.c_int),
+        "lb_reduce": ([H, dp], ctypes.c_int),
+        "lb_convergence": ([H, dp], ctypes.c_int),
+        "lb_upload_acc_device": ([H, ctypes.c_void_p], ctypes.c_int),
+        "lb_download_den_vel_device": ([H, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "lb_info": ([H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int),
+                     ctypes.POINTER(ctypes.c_uint64), dp], ctypes.c_int),
+        "lb_synchronize": ([H], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def check(rc):
+    """Map a C status code to the reference's exception types."""
+    if rc == LB_OK:
+        return
+    msg = lib().lb_last_error().decode(errors="replace")
+    if rc == LB_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Engine:
+    """Owner of one device solver handle (lb_solver*)."""
+
+    def __init__(self, lat, rlx, kvisc, g, bc, precision=LB_FP64, rho0=1.0, fp16_scale=0.0, layout=0):
+        L = lib()
+        if L.lb_device_count() < 1:
+            raise RuntimeError("jyulb: no HIP device available (the evolve path runs on the GPU only)")
+        cfg = LbConfig()
+        for a in range(3):
+            cfg.lat[a] = int(lat[a])
+            cfg.g[a] = float(g[a])
+        for f in range(6):
+            cfg.bc[f] = int(bc[f])
+        cfg.rlx_scheme = int(rlx)
+        cfg.kvisc = float(kvisc)
+        cfg.precision = int(PRECISIONS[precision])
+        cfg.rho0 = float(rho0)
+        cfg.fp16_scale = float(fp16_scale)
+        cfg.layout = int(layout)
+        h = ctypes.c_void_p()
+        check(L.lb_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.lat = tuple(int(v) for v in lat)
+        self.precision = cfg.precision
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().lb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        check(getattr(lib(), name)(self.h, *args))
+
+    def set_stream(self, stream_handle):
+        self.call("lb_set_stream", ctypes.c_void_p(stream_handle))
+
+    def set_gravity(self, g):
+        a = np.ascontiguousarray(g[:3], dtype=np.float64)
+        self.call("lb_set_gravity", a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def upload_phase(self, phase):
+        a = np.ascontiguousarray(phase, dtype=np.uint8)
+        self.call("lb_upload_phase", ptr(a))
+
+    def upload_den_vel(self, den, vel):
+        d = np.ascontiguousarray(den, dtype=np.float64)
+        v = np.ascontiguousarray(vel, dtype=np.float64)
+        self.call("lb_upload_den_vel", ptr(d), ptr(v))
+
+    def upload_acc(self, acc):
+        if acc is None:
+            self.call("lb_upload_acc", None)
+        else:
+            a = np.ascontiguousarray(acc, dtype=np.float64)
+            self.call("lb_upload_acc", ptr(a))
+
+    def set_face_data(self, face, den_face, vel_face):
+        d = np.ascontiguousarray(den_face, dtype=np.float64)
+        v = np.ascontiguousarray(vel_face, dtype=np.float64)
+        self.call("lb_set_face_data", int(face), ptr(d), ptr(v))
+
+    def finalize(self):
+        n = ctypes.c_int64()
+        self.call("lb_finalize", ctypes.byref(n))
+        return int(n.value)
+
+    def evolve(self, steps):
+        self.call("lb_evolve", ctypes.c_uint32(int(steps)))
+
+    def refresh_walls(self):
+        self.call("lb_refresh_walls")
+
+    def download_den_vel(self):
+        nx, ny, nz = self.lat
+        den = np.empty((nx, ny, nz))
+        vel = np.empty((nx, ny, nz, 3))
+        self.call("lb_download_den_vel", ptr(den), ptr(vel))
+        return den, vel
+
+    def download_post_f(self):
+        nx, ny, nz = self.lat
+        f = np.empty((19, nx, ny, nz))
+        self.call("lb_download_post_f", ptr(f))
+        return f
+
+    def reduce(self):
+        out = np.zeros(8)
+        self.call("lb_reduce", out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def convergence(self):
+        out = ctypes.c_double()
+        self.call("lb_convergence", ctypes.byref(out))
+        return out.value
+
+    def info(self):
+        st, odd, by, sc = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint64(), ctypes.c_double()
+        self.call("lb_info", ctypes.byref(st), ctypes.byref(odd), ctypes.byref(by), ctypes.byref(sc))
+        return dict(steps=st.value, next_odd=odd.value, device_bytes=by.value, fp16_scale=sc.value)
+
+    def synchronize(self):
+        self.call("lb_synchronize")

@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--layout", type=int, default=0, help="population layout (LB_LAYOUT_*)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a plain device copy of 2 GiB (torch copy_), the reference "
+                         "kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not supported "
@@ -124,6 +127,15 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU available", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate:
+        a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 2 GiB read + 2 GiB written", file=sys.stderr, flush=True)
+        del a, b
     precs = [p for p in args.precisions.split(",") if p]
     res = {}
     fields = {}

@@ -34,10 +34,10 @@ def sources():
     return out
 
 
-def build(verbose=False):
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, SRC]
+def build(verbose=False, out=OUT, defines=()):
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-o", tmp, SRC]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -46,8 +46,8 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed building libjyulb_hip.so")
     if verbose and r.stderr.strip():
         sys.stderr.write(r.stderr)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

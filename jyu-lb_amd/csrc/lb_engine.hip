@@ -67,19 +67,45 @@ static int fail(int code, const char* fmt, ...) {
 template <int P> struct Var;
 template <> struct Var<LB_FP64> { using TS = double; using TC = double; static constexpr bool SH = false; };
 template <> struct Var<LB_FP32_SHIFTED> { using TS = float; using TC = float; static constexpr bool SH = true; };
-template <> struct Var<LB_FP16_STORAGE> { using TS = __half; using TC = float; static constexpr bool SH = true; };
+template <> struct Var<LB_FP16_STORAGE> { using TS = _Float16; using TC = float; static constexpr bool SH = true; };
 
 template <class TC> LBD TC ldp(const double* p) { return (TC)(*p); }
 
-template <int P>
-LBD typename Var<P>::TC ldf(const typename Var<P>::TS* p, float inv_scale) {
-    if constexpr (P == LB_FP16_STORAGE) return __half2float(*p) * inv_scale;
-    else { (void)inv_scale; return *p; }
+// Cache policy of the population loads and stores: every population is
+// touched once per step, so streaming (non-temporal) accesses keep L2 for the
+// lines that neighbouring waves share. Separate switches per step parity and
+// direction (bit 0 even loads, 1 even stores, 2 odd loads, 3 odd stores;
+// bit 4: in odd steps stream the populations without a k component and
+// apply bits 2/3 only to the k-shifted ones).
+#ifndef LB_NT_MASK
+#define LB_NT_MASK 3
+#endif
+template <bool NT, class T> LBD T ld_stream(const T* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
-template <int P>
+template <bool NT, class T> LBD void st_stream(T* p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <int P, bool NT>
+LBD typename Var<P>::TC ldf(const typename Var<P>::TS* p, float inv_scale) {
+    if constexpr (P == LB_FP16_STORAGE) {
+        return (float)ld_stream<NT>(p) * inv_scale;
+    } else {
+        (void)inv_scale;
+        return ld_stream<NT>(p);
+    }
+}
+template <int P, bool NT>
 LBD void stf(typename Var<P>::TS* p, typename Var<P>::TC v, float scale) {
-    if constexpr (P == LB_FP16_STORAGE) *p = __float2half_rn(v * scale);
-    else { (void)scale; *p = v; }
+    if constexpr (P == LB_FP16_STORAGE)
+        st_stream<NT>(p, (_Float16)(v * scale));
+    else {
+        (void)scale;
+        st_stream<NT>(p, v);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -125,18 +151,23 @@ LBD bool resolve(const Dims& d, int& i, int& j, int& k, int& l, int& flip) {
     return true;
 }
 
+// A pointer known to be equal across the wave. Marking it keeps the compiler
+// from folding per-lane offsets into it early: the address of a load then
+// stays "scalar base + 32-bit lane offset" instead of one 64-bit vector
+// address per population held in registers until the store.
+template <class T>
+LBD T* uniform_ptr(T* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T*)(((uint64_t)hi << 32) | lo);
+}
+
 LBD void decompose(uint32_t c, int ny, int nzp, int& i, int& j, int& k) {
     uint32_t row = c / (uint32_t)nzp;
     k = (int)(c - row * (uint32_t)nzp);
     i = (int)(row / (uint32_t)ny);
     j = (int)(row - (uint32_t)i * (uint32_t)ny);
-}
-
-// blockIdx -> tile map that hands each XCD a contiguous range of tiles
-// (workgroups are dealt to the 8 XCDs round-robin); gridDim.x % 8 == 0.
-LBD uint32_t xcd_tile() {
-    const uint32_t per = gridDim.x >> 3;
-    return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -162,6 +193,10 @@ struct StepParams {
     int moving;
     int64_t rs;  // element stride between rows (i,j) of the population array
     int64_t ps;  // element stride between populations
+    const uint8_t* wclass;  // per wave of 64 sites: 0 = plain (see k_mask)
+    uint32_t nwaves;
+    const uint32_t* special;  // sites with M_SPECIAL, ascending within each wave
+    uint32_t nspecial;
 };
 
 // expanded index of padded site c in the population array: population l of
@@ -176,10 +211,11 @@ constexpr int TILE = 256;
 
 // element offset of the location an odd step reads f_l from (and writes
 // f*_opp(l) back to)
+template <bool SPECIAL = true>
 LBD int64_t odd_loc(const StepParams& p, uint32_t m, int64_t e, int l, int i, int j, int k) {
     if (l == REST) return e + (int64_t)REST * p.ps;
     if (m & (1u << l)) return e + (int64_t)l * p.ps;
-    if (!(m & M_BND))
+    if (!SPECIAL || !(m & M_BND))
         return e - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs - CK[l] + (int64_t)opp(l) * p.ps;
     int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
     resolve(p.d, yi, yj, yk, lp, fl);
@@ -223,37 +259,85 @@ LBD void ladd_all(const StepParams& p, uint32_t m, T* f, T cff1, T cff2, int i, 
     (ladd_if<T, Ls>(p, m, f, cff1, cff2, i, j, k), ...);
 }
 
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) void k_step(StepParams p) {
+#ifndef LB_PLAIN_PATH
+#define LB_PLAIN_PATH 1
+#endif
+#ifndef LB_WAVES_PER_EU
+#define LB_WAVES_PER_EU 0
+#endif
+#if LB_WAVES_PER_EU > 0
+#define LB_OCC __attribute__((amdgpu_waves_per_eu(LB_WAVES_PER_EU, 8)))
+#else
+#define LB_OCC
+#endif
+
+// One site of the fused step, in one of three modes:
+//   MODE_PLAIN    the site and its whole wave are interior fluid sites with no
+//                 bounce-back link: no link mask, loads issue immediately
+//   MODE_BULK     ordinary fluid site: bounce-back links from the link mask,
+//                 including Ladd's moving-wall term
+//   MODE_SPECIAL  sites with halo-crossing links (periodic / symmetric faces)
+//                 or on a Zou-He face; processed by a separate
+//                 kernel over a compact list so that these rarely taken paths
+//                 do not raise the register budget of the bulk kernel
+constexpr int MODE_PLAIN = 0, MODE_BULK = 1, MODE_SPECIAL = 2;
+
+template <int P, int RLX, bool ODD, int MODE>
+LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
+    constexpr bool PLAIN = MODE != MODE_SPECIAL;  // no special handling needed
     using TS = typename Var<P>::TS;
     using TC = typename Var<P>::TC;
     constexpr bool SH = Var<P>::SH;
-    const uint32_t tile = xcd_tile();
-    if (tile >= p.ntiles) return;
-    const uint32_t c = tile * TILE + threadIdx.x;
-    if (c >= p.nq) return;
-    const uint32_t m = p.mask[c];
-    if (!(m & M_FLUID)) return;
-
     TS* F = (TS*)p.f;
     const int64_t nq = p.nq;
     const int64_t e = eidx(p, c);
     int i = 0, j = 0, k = 0;
-    if (m & (M_BND | M_MOVING | M_ZH_ANY)) decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    if (MODE == MODE_SPECIAL || (MODE == MODE_BULK && (m & M_MOVING))) decompose(c, p.d.ny, p.d.nzp, i, j, k);
+
+    // Bulk sites address populations as a wave-uniform row base plus the
+    // unsigned lane offset kk (a wave never straddles rows: nzp % 64 == 0), so
+    // every load and store uses a scalar base and one shared 32-bit offset.
+    // Bounce-back links are exec-masked loads from the site's own slot.
+    const uint32_t nzp = (uint32_t)p.d.nzp;
+    const uint32_t row = PLAIN ? __builtin_amdgcn_readfirstlane(c / nzp) : 0;
+    const uint32_t kk = c - row * nzp;
+    TS* const Fr = F + (int64_t)row * p.rs;
+    auto own = [&](int l) { return uniform_ptr(Fr + (int64_t)l * p.ps); };
+    auto up = [&](int l) {
+        return uniform_ptr(Fr - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs + (int64_t)opp(l) * p.ps - CK[l]);
+    };
+    constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
+    constexpr bool NTLO = (LB_NT_MASK & 4) != 0, NTSO = (LB_NT_MASK & 8) != 0;
 
     // ---- stream (pull) ----
     TC f[Q];
-    if constexpr (!ODD) {
+    if constexpr (PLAIN && !ODD) {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) f[l] = ldf<P>(F + e + (int64_t)l * p.ps, p.inv_scale);
+        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l) + kk, p.inv_scale);
+    } else if constexpr (PLAIN) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            // k-shifted pulls straddle cache lines shared with the next wave:
+            // LB_NT_MASK bit 4 keeps those cached and streams the rest
+            constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTLO;
+            if (l == REST) f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
+            else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
+                if (CK[l] != 0) f[l] = ldf<P, NTLO>(up(l) + kk, p.inv_scale);
+                else f[l] = ldf<P, NTK>(up(l) + kk, p.inv_scale);
+            } else f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
+        }
+    } else if constexpr (!ODD) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(F + e + (int64_t)l * p.ps, p.inv_scale);
     } else {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) f[l] = ldf<P>(F + odd_loc(p, m, e, l, i, j, k), p.inv_scale);
+        for (int l = 0; l < Q; ++l)
+            f[l] = ldf<P, NTLO>(F + odd_loc<true>(p, m, e, l, i, j, k), p.inv_scale);
     }
     const TC rho0 = (TC)p.rho0;
 
     // ---- Ladd moving walls on bounce-back links (previous-step density) ----
-    if (p.moving && (m & M_MOVING)) {
+    if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) {
         const TC den = (TC)p.den[c];
         const TC cff1 = TC(2.0) * den * TC(2.0 / 36.0) * TC(3.0);
         const TC cff2 = TC(2.0) * den * TC(1.0 / 36.0) * TC(3.0);
@@ -261,7 +345,7 @@ __global__ __launch_bounds__(TILE) void k_step(StepParams p) {
     }
 
     // ---- Zou-He faces, order 1X, X1, 1Y, Y1, 1Z, Z1 (isothermal.pyx:1280-1291) ----
-    if (m & M_ZH_ANY) {
+    if (!PLAIN && (m & M_ZH_ANY)) {
 #pragma unroll
         for (int fc = 0; fc < 6; ++fc) {
             if (!(m & (M_ZH0 << fc))) continue;
@@ -289,18 +373,70 @@ __global__ __launch_bounds__(TILE) void k_step(StepParams p) {
         p.vel[c] = (double)vel[0];
         p.vel[nq + c] = (double)vel[1];
         p.vel[2 * nq + c] = (double)vel[2];
-    } else if (p.moving && (m & M_MOVING)) {
+    } else if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) {
         p.den[c] = (double)den;
     }
 
     // ---- store ----
-    if constexpr (!ODD) {
+    if constexpr (PLAIN && !ODD) {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) stf<P>(F + e + (int64_t)opp(l) * p.ps, f[l], p.scale);
+        for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)) + kk, f[l], p.scale);
+    } else if constexpr (PLAIN) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTSO;
+            if (l == REST) stf<P, NTK>(own(l) + kk, f[l], p.scale);
+            else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
+                if (CK[l] != 0) stf<P, NTSO>(up(l) + kk, f[opp(l)], p.scale);
+                else stf<P, NTK>(up(l) + kk, f[opp(l)], p.scale);
+            } else stf<P, NTK>(own(l) + kk, f[opp(l)], p.scale);
+        }
+    } else if constexpr (!ODD) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P, NTSE>(F + e + (int64_t)opp(l) * p.ps, f[l], p.scale);
     } else {
 #pragma unroll
-        for (int l = 0; l < Q; ++l) stf<P>(F + odd_loc(p, m, e, l, i, j, k), f[opp(l)], p.scale);
+        for (int l = 0; l < Q; ++l) stf<P, NTSO>(F + odd_loc<true>(p, m, e, l, i, j, k), f[opp(l)], p.scale);
     }
+}
+
+#ifndef LB_XCD_REMAP
+#define LB_XCD_REMAP 1
+#endif
+// Tile of this workgroup. With LB_XCD_REMAP the tiles are dealt so that each
+// of the 8 XCDs (which receive workgroups round-robin) works through one
+// contiguous range of rows; the grid is padded to a multiple of 8.
+LBD uint32_t tile_index() {
+    if (!LB_XCD_REMAP) return blockIdx.x;
+    const uint32_t per = gridDim.x >> 3;
+    return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+}
+
+// Bulk kernel: every fluid site that is not special.
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(TILE) LB_OCC void k_step(StepParams p) {
+    const uint32_t tile = tile_index();
+    if (tile >= p.ntiles) return;
+    const uint32_t c = tile * TILE + threadIdx.x;
+    // nq is a multiple of 64, so a wave is entirely inside or outside [0, nq)
+    const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
+    if (w >= p.nwaves) return;
+    if (LB_PLAIN_PATH && p.wclass[w] == 0) {
+        step_site<P, RLX, ODD, MODE_PLAIN>(p, c, M_FLUID);
+        return;
+    }
+    const uint32_t m = p.mask[c];
+    if ((m & (M_FLUID | M_SPECIAL)) != M_FLUID) return;
+    step_site<P, RLX, ODD, MODE_BULK>(p, c, m);
+}
+
+// Special sites, one thread per entry of the compact list.
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
+    const uint32_t t = blockIdx.x * TILE + threadIdx.x;
+    if (t >= p.nspecial) return;
+    const uint32_t c = p.special[t];
+    step_site<P, RLX, ODD, MODE_SPECIAL>(p, c, p.mask[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,40 +445,45 @@ __global__ __launch_bounds__(TILE) void k_step(StepParams p) {
 
 // Link mask of every site (see lb_lattice.h). `vel` (wall velocities) may be
 // null, then no site is flagged MOVING.
-__global__ void k_mask(const uint8_t* phase, const double* vel, uint32_t* mask, Dims d, uint32_t nq) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nq) return;
+// Link mask of every site (see lb_lattice.h) and the class of every wave of
+// 64 sites: 0 when all its sites are fluid with a bare M_FLUID mask (no
+// bounce-back, halo-crossing, Zou-He or moving-wall link), 1 otherwise.
+// `vel` (wall velocities) may be null, then no site is flagged MOVING.
+__global__ void k_mask(const uint8_t* phase, const double* vel, uint32_t* mask, uint8_t* wclass, Dims d,
+                       uint32_t nq) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;  // whole waves only: nq % 64 == 0
     int i, j, k;
     decompose(c, d.ny, d.nzp, i, j, k);
-    if (k >= d.nz || phase[c] == SOLID) {
-        mask[c] = 0;
-        return;
-    }
-    uint32_t m = M_FLUID;
-    for (int l = 0; l < Q; ++l) {
-        if (l == REST) continue;
-        int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
-        const bool crossed = yi < 0 || yi >= d.nx || yj < 0 || yj >= d.ny || yk < 0 || yk >= d.nz;
-        if (!resolve(d, yi, yj, yk, lp, fl)) {
-            m |= 1u << l;
-            continue;
+    uint32_t m = 0;
+    if (k < d.nz && phase[c] != SOLID) {
+        m = M_FLUID;
+        for (int l = 0; l < Q; ++l) {
+            if (l == REST) continue;
+            int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
+            const bool crossed = yi < 0 || yi >= d.nx || yj < 0 || yj >= d.ny || yk < 0 || yk >= d.nz;
+            if (!resolve(d, yi, yj, yk, lp, fl)) {
+                m |= 1u << l;
+                continue;
+            }
+            const int64_t y = ((int64_t)yi * d.ny + yj) * d.nzp + yk;
+            if (phase[y] == SOLID) {
+                m |= 1u << l;
+                if (vel && (vel[y] != 0.0 || vel[nq + y] != 0.0 || vel[2 * (int64_t)nq + y] != 0.0))
+                    m |= M_MOVING;
+            } else if (crossed) {
+                m |= M_BND;
+            }
         }
-        const int64_t y = ((int64_t)yi * d.ny + yj) * d.nzp + yk;
-        if (phase[y] == SOLID) {
-            m |= 1u << l;
-            if (vel && (vel[y] != 0.0 || vel[nq + y] != 0.0 || vel[2 * (int64_t)nq + y] != 0.0))
-                m |= M_MOVING;
-        } else if (crossed) {
-            m |= M_BND;
+        for (int fc = 0; fc < 6; ++fc) {
+            const bool fx = d.bc[fc] == FIXED_DEN || d.bc[fc] == FIXED_VEL;
+            const int a = fc >> 1, x = a == 0 ? i : a == 1 ? j : k, n = a == 0 ? d.nx : a == 1 ? d.ny : d.nz;
+            if (fx && x == ((fc & 1) ? n - 1 : 0)) m |= M_ZH0 << fc;
         }
     }
-    const bool fx[6] = {d.bc[0] == FIXED_DEN || d.bc[0] == FIXED_VEL, d.bc[1] == FIXED_DEN || d.bc[1] == FIXED_VEL,
-                        d.bc[2] == FIXED_DEN || d.bc[2] == FIXED_VEL, d.bc[3] == FIXED_DEN || d.bc[3] == FIXED_VEL,
-                        d.bc[4] == FIXED_DEN || d.bc[4] == FIXED_VEL, d.bc[5] == FIXED_DEN || d.bc[5] == FIXED_VEL};
-    const bool on[6] = {i == 0, i == d.nx - 1, j == 0, j == d.ny - 1, k == 0, k == d.nz - 1};
-    for (int fc = 0; fc < 6; ++fc)
-        if (fx[fc] && on[fc]) m |= M_ZH0 << fc;
     mask[c] = m;
+    const unsigned long long notplain = __ballot(m != M_FLUID);
+    if ((threadIdx.x & 63) == 0) wclass[c >> 6] = notplain ? 1 : 0;
 }
 
 // velocity of the site reached from x along +/-c_l for _comp_der_vel:
@@ -425,7 +566,7 @@ __global__ void k_init(StepParams p) {
     project<double, false>(h, f);
     TS* F = (TS*)p.f;
     const int64_t e = eidx(p, c);
-    for (int l = 0; l < Q; ++l) stf<P>(F + e + (int64_t)opp(l) * p.ps, (typename Var<P>::TC)f[l], p.scale);
+    for (int l = 0; l < Q; ++l) stf<P, false>(F + e + (int64_t)opp(l) * p.ps, (typename Var<P>::TC)f[l], p.scale);
 }
 
 // Post-collision populations of the newest step, unpacked into the reference's
@@ -446,7 +587,7 @@ __global__ void k_post_f(StepParams p, int next_odd, double* out) {
         if (m & M_FLUID) {
             const int64_t e = eidx(p, c);
             const int64_t loc = next_odd ? e + (int64_t)opp(l) * p.ps : odd_loc(p, m, e, opp(l), i, j, k);
-            v = (double)ldf<P>(F + loc, p.inv_scale);
+            v = (double)ldf<P, false>(F + loc, p.inv_scale);
             if (Var<P>::SH) v += (l == REST ? 12.0 / 36.0 : nnz(l) == 1 ? 2.0 / 36.0 : 1.0 / 36.0) * p.rho0;
         }
         out[(int64_t)l * nc + o] = v;
@@ -617,6 +758,9 @@ struct lb_solver {
     hipStream_t stream = nullptr;
     void* f = nullptr;
     uint32_t* mask = nullptr;
+    uint8_t* wclass = nullptr;
+    uint32_t* special = nullptr;  // compact list of special sites
+    uint32_t nspecial = 0, special_cap = 0;
     uint8_t* phase = nullptr;
     double* den = nullptr;
     double* vel = nullptr;
@@ -677,6 +821,11 @@ static StepParams params(const lb_solver* s) {
     p.ntiles = (s->nq + TILE - 1) / TILE;
     p.rs = s->rs;
     p.ps = s->ps;
+    p.wclass = s->wclass;
+    p.nwaves = s->nq / 64;
+    p.special = s->special;
+    p.nspecial = s->nspecial;
+
     p.moving = s->moving ? 1 : 0;
     return p;
 }
@@ -730,39 +879,76 @@ static int import_acc_from_stage(lb_solver* s) {
 }
 
 static int build_mask(lb_solver* s, bool with_walls) {
-    k_mask<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->phase, with_walls ? s->vel : nullptr, s->mask, s->d,
-                                                          s->nq);
+    k_mask<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->phase, with_walls ? s->vel : nullptr, s->mask,
+                                                          s->wclass, s->d, s->nq);
     HIPCHK(hipGetLastError());
     return LB_OK;
 }
 
-// count fluid sites and moving-wall sites from the mask
+// count fluid, moving-wall and special sites from the mask
 __global__ void k_count(const uint32_t* mask, uint32_t nq, unsigned long long* out) {
-    __shared__ unsigned long long sh[2];
-    if (threadIdx.x < 2) sh[threadIdx.x] = 0;
+    __shared__ unsigned long long sh[3];
+    if (threadIdx.x < 3) sh[threadIdx.x] = 0;
     __syncthreads();
-    unsigned long long a = 0, b = 0;
+    unsigned long long a = 0, b = 0, c3 = 0;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nq; c += gridDim.x * blockDim.x) {
         const uint32_t m = mask[c];
         a += (m & M_FLUID) ? 1 : 0;
         b += (m & M_MOVING) ? 1 : 0;
+        c3 += ((m & M_FLUID) && (m & M_SPECIAL)) ? 1 : 0;
     }
     atomicAdd(&sh[0], a);
     atomicAdd(&sh[1], b);
+    atomicAdd(&sh[2], c3);
     __syncthreads();
-    if (threadIdx.x < 2) atomicAdd(&out[threadIdx.x], sh[threadIdx.x]);
+    if (threadIdx.x < 3) atomicAdd(&out[threadIdx.x], sh[threadIdx.x]);
+}
+
+// compact list of the special fluid sites: one atomic per wave, ascending
+// order within a wave
+__global__ void k_list_special(const uint32_t* mask, uint32_t nq, uint32_t* list, unsigned int* counter) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;  // whole waves only: nq % 64 == 0
+    const uint32_t m = mask[c];
+    const bool sp = (m & M_FLUID) && (m & M_SPECIAL);
+    const unsigned long long b = __ballot(sp);
+    if (!b) return;
+    const int lane = threadIdx.x & 63;
+    unsigned int base = 0;
+    if (lane == 0) base = atomicAdd(counter, (unsigned int)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (sp) list[base + __popcll(b & ((1ull << lane) - 1ull))] = c;
 }
 
 static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
     unsigned long long* d = (unsigned long long*)s->red;
-    HIPCHK(hipMemsetAsync(d, 0, 16, s->stream));
+    HIPCHK(hipMemsetAsync(d, 0, 24, s->stream));
     k_count<<<256, 256, 0, s->stream>>>(s->mask, s->nq, d);
     HIPCHK(hipGetLastError());
-    unsigned long long h[2];
-    HIPCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, s->stream));
+    unsigned long long h[3];
+    HIPCHK(hipMemcpyAsync(h, d, 24, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     *nfluid = (int64_t)h[0];
     *nmoving = (int64_t)h[1];
+    const uint32_t nsp = (uint32_t)h[2];
+    if (nsp > s->special_cap) {
+        if (s->special) {
+            HIPCHK(hipFree(s->special));
+            s->bytes -= (size_t)s->special_cap * 4;
+            s->special = nullptr;
+        }
+        int rc = dalloc(s, (void**)&s->special, (size_t)nsp * 4);
+        if (rc) return rc;
+        s->special_cap = nsp;
+    }
+    s->nspecial = nsp;
+    if (nsp > 0) {
+        unsigned int* cnt = (unsigned int*)s->red;
+        HIPCHK(hipMemsetAsync(cnt, 0, 4, s->stream));
+        k_list_special<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->nq, s->special, cnt);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
     return LB_OK;
 }
 
@@ -790,9 +976,14 @@ static void launch_init(lb_solver* s) {
 
 template <int P, int RLX>
 static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
-    const unsigned grid = (p.ntiles + 7) / 8 * 8;
+    const unsigned grid = LB_XCD_REMAP ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
     if (odd) k_step<P, RLX, true><<<grid, TILE, 0, st>>>(p);
     else k_step<P, RLX, false><<<grid, TILE, 0, st>>>(p);
+    if (p.nspecial) {
+        const unsigned g2 = (p.nspecial + TILE - 1) / TILE;
+        if (odd) k_step_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
+        else k_step_special<P, RLX, false><<<g2, TILE, 0, st>>>(p);
+    }
 }
 
 template <int P>
@@ -864,6 +1055,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
                                                                                   : rows * s->rs;
     if ((rc = dalloc(s, &s->f, (size_t)s->fsize * ts_size(s->prec))) ||
         (rc = dalloc(s, (void**)&s->mask, nq * 4)) || (rc = dalloc(s, (void**)&s->phase, nq)) ||
+        (rc = dalloc(s, (void**)&s->wclass, nq / 64)) ||
         (rc = dalloc(s, (void**)&s->den, nq * 8)) || (rc = dalloc(s, (void**)&s->vel, 3 * nq * 8)) ||
         (rc = dalloc(s, (void**)&s->part, (size_t)RED_PARTS * RED_N * 8)) ||
         (rc = dalloc(s, (void**)&s->red, RED_N * 8))) {
@@ -875,6 +1067,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     // den = 1, vel = 0, phase SOLID, mask 0 (isothermal.pyx:243-248)
     (void)hipMemset(s->f, 0, (size_t)s->fsize * ts_size(s->prec));
     (void)hipMemset(s->mask, 0, nq * 4);
+    (void)hipMemset(s->wclass, 1, nq / 64);
     (void)hipMemset(s->phase, 0, nq);
     (void)hipMemset(s->vel, 0, 3 * nq * 8);
     std::vector<double> ones;  // den = 1 via a host copy of one row block
@@ -909,7 +1102,7 @@ int lb_destroy(lb_solver* s) {
     if (!s) return LB_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     else (void)hipDeviceSynchronize();
-    void* ptrs[] = {s->f, s->mask, s->phase, s->den, s->vel, s->acc, s->stage, s->part, s->red, s->uprev};
+    void* ptrs[] = {s->f, s->mask, s->wclass, s->special, s->phase, s->den, s->vel, s->acc, s->stage, s->part, s->red, s->uprev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {

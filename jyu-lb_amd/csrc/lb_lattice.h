@@ -62,5 +62,7 @@ constexpr uint32_t M_MOVING = 1u << 21;
 constexpr uint32_t M_ZH0 = 1u << 22;  // faces 0..5 -> bits 22..27
 constexpr uint32_t M_ZH_ANY = 0x3Fu << 22;
 constexpr uint32_t M_LINKS = (1u << 19) - 1;
+// sites handled by the special-site kernel (see lb_engine.hip)
+constexpr uint32_t M_SPECIAL = M_BND | M_ZH_ANY;
 
 }  // namespace lb

@@ -131,6 +131,36 @@ int lb_convergence(lb_solver *s, double *rel_change);
 int lb_upload_acc_device(lb_solver *s, const double *acc_dev);
 int lb_download_den_vel_device(lb_solver *s, double *den_dev, double *vel_dev);
 
+/* Zou-He face data from device memory (same layout as lb_set_face_data). */
+int lb_set_face_data_device(lb_solver *s, int face, const double *den_dev, const double *vel_dev);
+
+/* Objective of the adjoint, evaluated on the last stored den/vel: mean over
+ * the fluid sites of the face layer `face` of rho * u*_axis (the mass flux
+ * through that layer, e.g. face X1, axis 0 = mean outlet x-flux). FP64. */
+int lb_objective(lb_solver *s, int face, int axis, double *J);
+
+/* Adjoint, forward pass (no counterpart in the reference: reverse-mode
+ * differentiation of `steps` evolve steps with fixed parameters). Runs the
+ * steps like lb_evolve, keeping a checkpoint of the population array every
+ * `interval` steps, and returns the objective J (lb_objective) after the last
+ * step. FP64 or FP32 precision; stationary walls only. */
+int lb_adj_forward(lb_solver *s, uint32_t steps, uint32_t interval, int face, int axis, double *J);
+
+/* Adjoint, reverse pass: gradients of J from the last lb_adj_forward with
+ * respect to the per-site acceleration field (f64 [nx][ny][nz][3]) and to the
+ * velocity data of Zou-He face `face` (f64, face layout x 3; face < 0: none).
+ * Each checkpoint segment is recomputed forward, recording (rho, u*) per site
+ * and step, then stepped back with the adjoint AA step. The population array
+ * is left in its state after the forward pass. Null outputs are skipped. */
+int lb_adj_backward(lb_solver *s, int face, double *grad_acc, double *grad_face);
+int lb_adj_backward_device(lb_solver *s, int face, double *grad_acc_dev, double *grad_face_dev);
+
+/* One snapshot slot of the device state (populations, den/vel, step count);
+ * lb_restore returns to it. Used for finite-difference checks and for
+ * re-running a segment (no counterpart in the reference). */
+int lb_snapshot(lb_solver *s);
+int lb_restore(lb_solver *s);
+
 /* Introspection: number of AA steps taken, parity of the next step (1 = odd),
  * bytes of device memory held, storage scale of the FP16 variant. */
 int lb_info(lb_solver *s, uint64_t *steps, int *next_odd, uint64_t *device_bytes, double *fp16_scale);

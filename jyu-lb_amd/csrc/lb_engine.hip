@@ -35,6 +35,7 @@
 #include "../../include/jyulb_gpu.h"
 #include "lb_lattice.h"
 #include "lb_physics.h"
+#include "lb_adjoint.h"
 
 using namespace lb;
 
@@ -197,6 +198,7 @@ struct StepParams {
     uint32_t nwaves;
     const uint32_t* special;  // sites with M_SPECIAL, ascending within each wave
     uint32_t nspecial;
+    void* rec;  // if set: (rho, u*) of this step, 4 * nq of the compute type (adjoint recompute)
 };
 
 // expanded index of padded site c in the population array: population l of
@@ -208,6 +210,7 @@ LBD int64_t eidx(const StepParams& p, uint32_t c) {
 }
 
 constexpr int TILE = 256;
+constexpr int RED_BLOCK_ADJ = 256;
 
 // element offset of the location an odd step reads f_l from (and writes
 // f*_opp(l) back to)
@@ -376,6 +379,13 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     } else if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) {
         p.den[c] = (double)den;
     }
+    if (p.rec) {
+        TC* R = (TC*)p.rec;
+        R[c] = den;
+        R[nq + c] = vel[0];
+        R[2 * nq + c] = vel[1];
+        R[3 * nq + c] = vel[2];
+    }
 
     // ---- store ----
     if constexpr (PLAIN && !ODD) {
@@ -437,6 +447,149 @@ __global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
     if (t >= p.nspecial) return;
     const uint32_t c = p.special[t];
     step_site<P, RLX, ODD, MODE_SPECIAL>(p, c, p.mask[c]);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint step
+// ---------------------------------------------------------------------------
+// The adjoint of an AA step is an AA step of the same parity on the cotangent
+// array: site x reads the cotangents of its post-collision populations from
+// the locations its forward step wrote them to, pulls them back through the
+// collision and the Zou-He faces (reverse order) and writes the cotangents of
+// its pre-collision populations to the locations its forward step read them
+// from. Those are the same 19 locations, so the step runs in place.
+struct AdjParams {
+    void* lam;          // cotangent array, population layout of f, compute type
+    const void* rec;    // (rho, u*) recorded by the forward recompute of this step
+    void* gacc;         // accumulated acceleration cotangent, 3 * nq, compute type
+    double* gface;      // accumulated face-velocity cotangent (face of gface_id), face layout x 3
+    int gface_id;
+    int seed;           // 1 on the objective's step
+    int obj_face, obj_axis;
+    double obj_scale;   // 1 / (fluid sites on the objective face)
+};
+
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
+    using TC = typename Var<P>::TC;
+    const uint32_t c = blockIdx.x * TILE + threadIdx.x;
+    if (c >= p.nq) return;
+    const uint32_t m = p.mask[c];
+    if (!(m & M_FLUID)) return;
+    const int64_t nq = p.nq;
+    const int64_t e = eidx(p, c);
+    int i, j, k;
+    decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    TC* L = (TC*)x.lam;
+    TC lam[Q], mu[Q];
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        if (ODD) lam[opp(l)] = L[odd_loc<true>(p, m, e, l, i, j, k)];
+        else lam[l] = L[e + (int64_t)opp(l) * p.ps];
+    }
+    const TC* R = (const TC*)x.rec;
+    const TC rho = R[c];
+    const TC us[3] = {R[nq + c], R[2 * nq + c], R[3 * nq + c]};
+    TC acc[3];
+    const TC* A = (const TC*)p.acc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
+    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
+    TC ga[3] = {TC(0), TC(0), TC(0)};
+    collide_vjp<RLX, TC>(lam, rho, us, acc, prm, mu, ga);
+    if (x.seed) {
+        // objective: mean over the face layer of rho u*_axis = j_axis + rho a_axis / 2
+        const int ax = x.obj_face >> 1;
+        const int pos = ax == 0 ? i : ax == 1 ? j : k;
+        const int n = ax == 0 ? p.d.nx : ax == 1 ? p.d.ny : p.d.nz;
+        if (pos == ((x.obj_face & 1) ? n - 1 : 0)) {
+            const TC sc = (TC)x.obj_scale;
+#pragma unroll
+            for (int l = 0; l < Q; ++l) mu[l] += sc * (TC(ccomp(l, x.obj_axis)) + TC(0.5) * acc[x.obj_axis]);
+            ga[x.obj_axis] += sc * TC(0.5) * rho;
+        }
+    }
+    if (x.gacc) {
+        TC* G = (TC*)x.gacc;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) G[(int64_t)a * nq + c] += ga[a];
+    }
+    if (m & M_ZH_ANY) {
+        for (int fc = 5; fc >= 0; --fc) {
+            if (!(m & (M_ZH0 << fc))) continue;
+            int64_t fi;
+            if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
+            else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
+            else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
+            const TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
+            const int kind = p.d.bc[fc];
+            const TC dbc = kind == FIXED_DEN ? (TC)p.fden[fc][fi] : rho;
+            TC nu[Q], gv[3] = {TC(0), TC(0), TC(0)};
+            zh_vjp<TC>(fc, kind, dbc, vb, mu, nu, gv);
+#pragma unroll
+            for (int l = 0; l < Q; ++l) mu[l] = nu[l];
+            if (x.gface && fc == x.gface_id)
+                for (int a = 0; a < 3; ++a) x.gface[3 * fi + a] += (double)gv[a];
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        if (ODD) L[odd_loc<true>(p, m, e, l, i, j, k)] = mu[l];
+        else L[e + (int64_t)l * p.ps] = mu[l];
+    }
+}
+
+// objective: sum over fluid sites of the face layer of rho u*_axis (FP64)
+__global__ __launch_bounds__(RED_BLOCK_ADJ) void k_obj(const uint32_t* mask, const double* den, const double* vel,
+                                                   Dims d, uint32_t nq, int face, int axis, double* part) {
+    double a = 0.0, n = 0.0;
+    const int ax = face >> 1;
+    for (uint32_t c = blockIdx.x * RED_BLOCK_ADJ + threadIdx.x; c < nq; c += gridDim.x * RED_BLOCK_ADJ) {
+        if (!(mask[c] & M_FLUID)) continue;
+        int i, j, k;
+        decompose(c, d.ny, d.nzp, i, j, k);
+        const int pos = ax == 0 ? i : ax == 1 ? j : k;
+        const int nn = ax == 0 ? d.nx : ax == 1 ? d.ny : d.nz;
+        if (pos != ((face & 1) ? nn - 1 : 0)) continue;
+        a += den[c] * vel[(int64_t)axis * nq + c];
+        n += 1.0;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        n += __shfl_xor(n, o, 64);
+    }
+    __shared__ double sh[RED_BLOCK_ADJ / 64][2];
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6][0] = a;
+        sh[threadIdx.x >> 6][1] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < RED_BLOCK_ADJ / 64; ++w) {
+            s0 += sh[w][0];
+            s1 += sh[w][1];
+        }
+        part[2 * blockIdx.x] = s0;
+        part[2 * blockIdx.x + 1] = s1;
+    }
+}
+
+template <class TC>
+__global__ void k_export_vec3(const TC* src, double* out, Dims d, uint32_t nq) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    if (k >= d.nz) return;
+    const int64_t o = ((int64_t)i * d.ny + j) * d.nz + k;
+    for (int a = 0; a < 3; ++a) out[3 * o + a] = (double)src[(int64_t)a * nq + c];
+}
+
+template <class T>
+__global__ void k_fill(T* p, size_t n, T v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -779,6 +932,25 @@ struct lb_solver {
     uint64_t steps = 0;
     int next_odd = 1;
     uint64_t bytes = 0;
+    // adjoint (checkpointed reverse pass)
+    void* ckpt = nullptr;  // (nseg + 1) states: segment starts and the final state
+    size_t ckpt_bytes = 0;
+    void* lam = nullptr;
+    void* rec = nullptr;
+    size_t rec_bytes = 0;
+    void* gacc = nullptr;
+    double* gface = nullptr;
+    uint32_t adj_steps = 0, adj_interval = 0;
+    int adj_po = 1;
+    uint64_t adj_step_end = 0;
+    int adj_face = FX1, adj_axis = 0;
+    double adj_scale = 0.0;
+    bool adj_ready = false;
+    // one snapshot slot (lb_snapshot / lb_restore)
+    void* snap = nullptr;
+    double* snap_dv = nullptr;
+    uint64_t snap_steps = 0;
+    int snap_odd = 1;
 };
 
 static constexpr int RED_PARTS = 1024;
@@ -993,6 +1165,28 @@ static void launch_step_rlx(const StepParams& p, int rlx, bool odd, hipStream_t 
     else launch_step<P, REGUL>(p, odd, st);
 }
 
+static void launch_any_step(const lb_solver* s, const StepParams& p, bool odd) {
+    switch (s->prec) {
+    case LB_FP64: launch_step_rlx<LB_FP64>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+    case LB_FP32_SHIFTED: launch_step_rlx<LB_FP32_SHIFTED>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+    default: launch_step_rlx<LB_FP16_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+    }
+}
+
+template <int P, int RLX>
+static void launch_adj(const lb_solver* s, const StepParams& p, const AdjParams& x, bool odd) {
+    const unsigned grid = (s->nq + TILE - 1) / TILE;
+    if (odd) k_adj<P, RLX, true><<<grid, TILE, 0, s->stream>>>(p, x);
+    else k_adj<P, RLX, false><<<grid, TILE, 0, s->stream>>>(p, x);
+}
+
+template <int P>
+static void launch_adj_rlx(const lb_solver* s, const StepParams& p, const AdjParams& x, bool odd) {
+    if (s->cfg.rlx_scheme == BGK) launch_adj<P, BGK>(s, p, x, odd);
+    else if (s->cfg.rlx_scheme == TRT) launch_adj<P, TRT>(s, p, x, odd);
+    else launch_adj<P, REGUL>(s, p, x, odd);
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1102,7 +1296,9 @@ int lb_destroy(lb_solver* s) {
     if (!s) return LB_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     else (void)hipDeviceSynchronize();
-    void* ptrs[] = {s->f, s->mask, s->wclass, s->special, s->phase, s->den, s->vel, s->acc, s->stage, s->part, s->red, s->uprev};
+    void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
+                    s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
+                    s->snap,  s->snap_dv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {
@@ -1333,6 +1529,205 @@ int lb_convergence(lb_solver* s, double* rel) {
     } else {
         *rel = h[1] > 0 ? sqrt(h[0] / h[1]) : (h[0] > 0 ? INFINITY : 0.0);
     }
+    return LB_OK;
+}
+
+static int objective(lb_solver* s, int face, int axis, double* J, double* count) {
+    const int nb = 256;
+    k_obj<<<nb, RED_BLOCK_ADJ, 0, s->stream>>>(s->mask, s->den, s->vel, s->d, s->nq, face, axis, s->part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(2 * nb);
+    HIPCHK(hipMemcpyAsync(h.data(), s->part, 2 * nb * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double a = 0.0, n = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        a += h[2 * b];
+        n += h[2 * b + 1];
+    }
+    if (count) *count = n;
+    *J = n > 0 ? a / n : 0.0;
+    return LB_OK;
+}
+
+int lb_objective(lb_solver* s, int face, int axis, double* J) {
+    if (!s || !J || face < 0 || face > 5 || axis < 0 || axis > 2) return fail(LB_EINVAL, "invalid argument");
+    return objective(s, face, axis, J, nullptr);
+}
+
+int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, int axis, double* J) {
+    if (!s || !J || face < 0 || face > 5 || axis < 0 || axis > 2) return fail(LB_EINVAL, "invalid argument");
+    if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
+    if (steps < 1 || interval < 1) return fail(LB_EINVAL, "steps and checkpoint interval must be >= 1");
+    if (s->prec == LB_FP16_STORAGE) return fail(LB_EINVAL, "the adjoint runs in FP64 or FP32 (not FP16 storage)");
+    if (s->moving) return fail(LB_EINVAL, "the adjoint supports stationary walls only (no moving solid sites)");
+    const uint32_t nseg = (steps + interval - 1) / interval;
+    const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
+    int rc;
+    if (s->ckpt_bytes < (size_t)(nseg + 1) * fbytes) {
+        if (s->ckpt) {
+            HIPCHK(hipFree(s->ckpt));
+            s->bytes -= s->ckpt_bytes;
+            s->ckpt = nullptr;
+        }
+        if ((rc = dalloc(s, &s->ckpt, (size_t)(nseg + 1) * fbytes))) return rc;
+        s->ckpt_bytes = (size_t)(nseg + 1) * fbytes;
+    }
+    s->adj_po = s->next_odd;
+    StepParams p = params(s);
+    for (uint32_t seg = 0; seg < nseg; ++seg) {
+        HIPCHK(hipMemcpyAsync((char*)s->ckpt + (size_t)seg * fbytes, s->f, fbytes, hipMemcpyDeviceToDevice, s->stream));
+        const uint32_t len = std::min(interval, steps - seg * interval);
+        for (uint32_t t = 0; t < len; ++t) {
+            p.store_dv = (seg * interval + t + 1 == steps) ? 1 : 0;
+            launch_any_step(s, p, s->next_odd != 0);
+            s->next_odd ^= 1;
+            s->steps++;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync((char*)s->ckpt + (size_t)nseg * fbytes, s->f, fbytes, hipMemcpyDeviceToDevice, s->stream));
+    double n = 0.0;
+    if ((rc = objective(s, face, axis, J, &n))) return rc;
+    if (n <= 0) return fail(LB_EINVAL, "objective face %d has no fluid site", face);
+    s->adj_steps = steps;
+    s->adj_interval = interval;
+    s->adj_step_end = s->steps;
+    s->adj_face = face;
+    s->adj_axis = axis;
+    s->adj_scale = 1.0 / n;
+    s->adj_ready = true;
+    return LB_OK;
+}
+
+static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* grad_face, bool device) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->adj_ready) return fail(LB_ESTATE, "lb_adj_forward must precede lb_adj_backward");
+    if (gface_id >= 0 && (gface_id > 5 || !s->fden[gface_id]))
+        return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", gface_id);
+    const size_t tcs = tc_size(s->prec);
+    const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
+    const size_t recb = (size_t)s->adj_interval * 4 * s->nq * tcs;
+    int rc;
+    if (!s->lam && (rc = dalloc(s, &s->lam, (size_t)s->fsize * tcs))) return rc;
+    if (!s->gacc && (rc = dalloc(s, &s->gacc, (size_t)3 * s->nq * tcs))) return rc;
+    const size_t nface = (size_t)(s->NI > s->NJ ? s->NI : s->NJ) * (s->NJ > s->NK ? s->NJ : s->NK) * 3;
+    if (!s->gface && (rc = dalloc(s, (void**)&s->gface, std::max<size_t>(nface, 3 * (size_t)s->NI * s->NJ) * 8)))
+        return rc;
+    if (s->rec_bytes < recb) {
+        if (s->rec) {
+            HIPCHK(hipFree(s->rec));
+            s->bytes -= s->rec_bytes;
+            s->rec = nullptr;
+        }
+        if ((rc = dalloc(s, &s->rec, recb))) return rc;
+        s->rec_bytes = recb;
+    }
+    const size_t fface = gface_id < 0 ? 0
+                         : (size_t)(gface_id < 2 ? s->NJ * s->NK : gface_id < 4 ? s->NI * s->NK : s->NI * s->NJ);
+    const unsigned fb = 256;
+    if (s->prec == LB_FP64) {
+        k_fill<double><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->lam, s->fsize, 0.0);
+        k_fill<double><<<(unsigned)((3 * (size_t)s->nq + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->gacc, 3 * (size_t)s->nq, 0.0);
+    } else {
+        k_fill<float><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->lam, s->fsize, 0.0f);
+        k_fill<float><<<(unsigned)((3 * (size_t)s->nq + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->gacc, 3 * (size_t)s->nq, 0.0f);
+    }
+    if (fface) k_fill<double><<<(unsigned)((3 * fface + fb - 1) / fb), fb, 0, s->stream>>>(s->gface, 3 * fface, 0.0);
+    HIPCHK(hipGetLastError());
+    const uint32_t steps = s->adj_steps, interval = s->adj_interval;
+    const uint32_t nseg = (steps + interval - 1) / interval;
+    StepParams p = params(s);
+    p.store_dv = 0;
+    AdjParams x;
+    x.lam = s->lam;
+    x.gacc = s->gacc;
+    x.gface = fface ? s->gface : nullptr;
+    x.gface_id = gface_id;
+    x.obj_face = s->adj_face;
+    x.obj_axis = s->adj_axis;
+    x.obj_scale = s->adj_scale;
+    for (int seg = (int)nseg - 1; seg >= 0; --seg) {
+        HIPCHK(hipMemcpyAsync(s->f, (char*)s->ckpt + (size_t)seg * fbytes, fbytes, hipMemcpyDeviceToDevice, s->stream));
+        const uint32_t g0 = (uint32_t)seg * interval;
+        const uint32_t len = std::min(interval, steps - g0);
+        for (uint32_t t = 0; t < len; ++t) {  // recompute, recording (rho, u*)
+            p.rec = (char*)s->rec + (size_t)t * 4 * s->nq * tcs;
+            launch_any_step(s, p, (s->adj_po ^ ((g0 + t) & 1)) != 0);
+        }
+        p.rec = nullptr;
+        for (int t = (int)len - 1; t >= 0; --t) {
+            const uint32_t g = g0 + (uint32_t)t;
+            x.rec = (const char*)s->rec + (size_t)t * 4 * s->nq * tcs;
+            x.seed = (g + 1 == steps) ? 1 : 0;
+            const bool odd = (s->adj_po ^ (g & 1)) != 0;
+            if (s->prec == LB_FP64) launch_adj_rlx<LB_FP64>(s, p, x, odd);
+            else launch_adj_rlx<LB_FP32_SHIFTED>(s, p, x, odd);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    // restore the state after the forward pass
+    HIPCHK(hipMemcpyAsync(s->f, (char*)s->ckpt + (size_t)nseg * fbytes, fbytes, hipMemcpyDeviceToDevice, s->stream));
+    if ((rc = ensure_stage(s))) return rc;
+    if (grad_acc) {
+        if (s->prec == LB_FP64)
+            k_export_vec3<double><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>((const double*)s->gacc, s->stage, s->d, s->nq);
+        else
+            k_export_vec3<float><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>((const float*)s->gacc, s->stage, s->d, s->nq);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(grad_acc, s->stage, (size_t)s->ncell * 24,
+                              device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s->stream));
+    }
+    if (grad_face && fface)
+        HIPCHK(hipMemcpyAsync(grad_face, s->gface, fface * 24, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                              s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_adj_backward(lb_solver* s, int face, double* grad_acc, double* grad_face) {
+    return adj_backward(s, face, grad_acc, grad_face, false);
+}
+
+int lb_adj_backward_device(lb_solver* s, int face, double* grad_acc_dev, double* grad_face_dev) {
+    return adj_backward(s, face, grad_acc_dev, grad_face_dev, true);
+}
+
+int lb_set_face_data_device(lb_solver* s, int face, const double* den_dev, const double* vel_dev) {
+    if (!s || face < 0 || face > 5 || !den_dev || !vel_dev) return fail(LB_EINVAL, "invalid argument");
+    if (!s->fden[face]) return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", face);
+    const size_t n = (size_t)(face < 2 ? s->NJ * s->NK : face < 4 ? s->NI * s->NK : s->NI * s->NJ);
+    HIPCHK(hipMemcpyAsync(s->fden[face], den_dev, n * 8, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->fvel[face], vel_dev, 3 * n * 8, hipMemcpyDeviceToDevice, s->stream));
+    return LB_OK;
+}
+
+int lb_snapshot(lb_solver* s) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
+    const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
+    int rc;
+    if (!s->snap && (rc = dalloc(s, &s->snap, fbytes))) return rc;
+    if (!s->snap_dv && (rc = dalloc(s, (void**)&s->snap_dv, (size_t)4 * s->nq * 8))) return rc;
+    HIPCHK(hipMemcpyAsync(s->snap, s->f, fbytes, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->snap_dv, s->den, (size_t)s->nq * 8, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->snap_dv + s->nq, s->vel, (size_t)3 * s->nq * 8, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->snap_steps = s->steps;
+    s->snap_odd = s->next_odd;
+    return LB_OK;
+}
+
+int lb_restore(lb_solver* s) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->snap) return fail(LB_ESTATE, "no snapshot taken");
+    const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
+    HIPCHK(hipMemcpyAsync(s->f, s->snap, fbytes, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->den, s->snap_dv, (size_t)s->nq * 8, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->vel, s->snap_dv + s->nq, (size_t)3 * s->nq * 8, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->steps = s->snap_steps;
+    s->next_odd = s->snap_odd;
+    s->adj_ready = false;
     return LB_OK;
 }
 

@@ -68,6 +68,13 @@ def lib():
         "lb_info": ([H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int),
                      ctypes.POINTER(ctypes.c_uint64), dp], ctypes.c_int),
         "lb_synchronize": ([H], ctypes.c_int),
+        "lb_set_face_data_device": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "lb_objective": ([H, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
+        "lb_adj_forward": ([H, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
+        "lb_adj_backward": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "lb_adj_backward_device": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "lb_snapshot": ([H], ctypes.c_int),
+        "lb_restore": ([H], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -199,3 +206,42 @@ class Engine:
 
     def synchronize(self):
         self.call("lb_synchronize")
+
+    def face_shape(self, face):
+        nx, ny, nz = self.lat
+        return (ny + 2, nz + 2) if face < 2 else (nx + 2, nz + 2) if face < 4 else (nx + 2, ny + 2)
+
+    def set_face_data_device(self, face, den_ptr, vel_ptr):
+        self.call("lb_set_face_data_device", int(face), ctypes.c_void_p(den_ptr), ctypes.c_void_p(vel_ptr))
+
+    def objective(self, face, axis):
+        out = ctypes.c_double()
+        self.call("lb_objective", int(face), int(axis), ctypes.byref(out))
+        return out.value
+
+    def adj_forward(self, steps, interval, face, axis):
+        out = ctypes.c_double()
+        self.call("lb_adj_forward", ctypes.c_uint32(int(steps)), ctypes.c_uint32(int(interval)),
+                  int(face), int(axis), ctypes.byref(out))
+        return out.value
+
+    def adj_backward(self, grad_face=-1):
+        """Host gradients: (grad_acc (nx,ny,nz,3), grad_face_vel (face shape,3) or None)."""
+        nx, ny, nz = self.lat
+        ga = np.empty((nx, ny, nz, 3))
+        gf = np.empty(self.face_shape(grad_face) + (3,)) if grad_face >= 0 else None
+        self.call("lb_adj_backward", int(grad_face), ptr(ga), ptr(gf) if gf is not None else None)
+        return ga, gf
+
+    def adj_backward_device(self, grad_face, ga_ptr, gf_ptr):
+        self.call("lb_adj_backward_device", int(grad_face), ctypes.c_void_p(ga_ptr),
+                  ctypes.c_void_p(gf_ptr) if gf_ptr else None)
+
+    def upload_acc_device(self, acc_ptr):
+        self.call("lb_upload_acc_device", ctypes.c_void_p(acc_ptr))
+
+    def snapshot(self):
+        self.call("lb_snapshot")
+
+    def restore(self):
+        self.call("lb_restore")

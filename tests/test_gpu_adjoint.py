@@ -1,0 +1,161 @@
+"""Adjoint AA step vs central finite differences of the forward model.
+
+Objective: mean outlet x-flux J = <rho u*_x> over the fluid sites of face X1
+after `steps` steps. Gradients with respect to the per-site acceleration field
+(sampled sites, all components) and to the Zou-He inlet velocity data
+(through the amplitude A of the inlet profile, dJ/dA = sum dJ/dv * profile).
+Finite differences: FP64 forward on the GPU from the same initial state,
+central, h = 1e-6.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from jyulb.defs import BGK, TRT, REGULARIZATION, WALL, PERIODIC, FIXED_DEN, FIXED_VEL, SYMMETRIC
+from jyulb.defs import FACE_1X, FACE_X1, X
+
+
+def make_channel(rlx, precision, lat=(14, 9, 7), steps_bc=(WALL, SYMMETRIC), seed=3, solid=0.08):
+    from jyulb.isothermal import Solver
+    nx, ny, nz = lat
+    bc = np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, steps_bc[0], steps_bc[1]], dtype=np.int32)
+    if steps_bc[0] == PERIODIC:
+        bc[4] = bc[5] = PERIODIC
+    s = Solver(np.array(lat, dtype=np.int32), rlx, 0.2 / 3.0, np.array([1e-5, 0.0, 0.0]), bc,
+               precision=precision)
+    rng = np.random.default_rng(seed)
+    ph = s.get_phase()
+    ph[...] = 1
+    core = rng.random(lat) < solid
+    core[0, :, :] = False
+    core[-1, :, :] = False
+    ph[core] = 0
+    prof = np.zeros((ny + 2, nz + 2))
+    eta_y = (np.arange(ny) + 0.5) / ny - 0.5
+    eta_z = (np.arange(nz) + 0.5) / nz - 0.5
+    prof[1:-1, 1:-1] = (1 - (2 * eta_y[:, None]) ** 2) * (1 - (2 * eta_z[None, :]) ** 2)
+    amp = 0.02
+    s.get_vel()[0, :, :, 0] = amp * prof[1:-1, 1:-1]
+    s.get_acc()[...] = 2e-6 * (rng.random(lat + (3,)) - 0.5)
+    s.finalize_init()
+    return s, prof, amp
+
+
+def _inputs(s):
+    s._set_bc_data()
+    return (np.ascontiguousarray(s.get_acc()).copy(), s.bc_data_den_face_1X.copy(),
+            s.bc_data_vel_face_1X.copy(), np.array(s.gravity))
+
+
+def _J(eng, acc, fden, fvel, g, steps):
+    eng.restore()
+    eng.set_gravity(g)
+    eng.upload_acc(acc)
+    eng.set_face_data(FACE_1X, fden, fvel)
+    eng.evolve(steps)
+    return eng.objective(FACE_X1, X)
+
+
+def _adjoint(s, acc, fden, fvel, g, steps, interval):
+    eng = s.engine
+    eng.restore()
+    eng.set_gravity(g)
+    eng.upload_acc(acc)
+    eng.set_face_data(FACE_1X, fden, fvel)
+    J = eng.adj_forward(steps, interval, FACE_X1, X)
+    ga, gf = eng.adj_backward(FACE_1X)
+    return J, ga, gf
+
+
+def _fd_checks(s_adj, s_fd, prof, amp, steps, interval, ncells, seed):
+    acc, fden, fvel, g = _inputs(s_fd)
+    s_fd.engine.snapshot()
+    s_adj.engine.snapshot()
+    J, ga, gf = _adjoint(s_adj, acc, fden, fvel, g, steps, interval)
+    J0 = _J(s_fd.engine, acc, fden, fvel, g, steps)
+    h = 1e-6
+    rng = np.random.default_rng(seed)
+    fl = np.argwhere(s_fd.get_phase() != 0)
+    pick = fl[rng.choice(len(fl), size=ncells, replace=False)]
+    adj, fd = [], []
+    for (i, j, k) in pick:
+        a = int(rng.integers(0, 3))
+        ap = acc.copy()
+        ap[i, j, k, a] += h
+        am = acc.copy()
+        am[i, j, k, a] -= h
+        fd.append((_J(s_fd.engine, ap, fden, fvel, g, steps) - _J(s_fd.engine, am, fden, fvel, g, steps)) / (2 * h))
+        adj.append(ga[i, j, k, a])
+    # inlet amplitude: v = A * profile (x component)
+    vp = fvel.copy()
+    vp[:, :, 0] = (amp + h) * prof
+    vm = fvel.copy()
+    vm[:, :, 0] = (amp - h) * prof
+    fdA = (_J(s_fd.engine, acc, fden, vp, g, steps) - _J(s_fd.engine, acc, fden, vm, g, steps)) / (2 * h)
+    adjA = float(np.sum(gf[:, :, 0] * prof))
+    return J, J0, np.array(adj), np.array(fd), adjA, fdA
+
+
+@pytest.mark.parametrize("rlx", [BGK, TRT, REGULARIZATION], ids=["bgk", "trt", "reg"])
+def test_adjoint_fp64_matches_finite_differences(rlx):
+    print(f"adjoint fp64 rlx={rlx}", flush=True)
+    s, prof, amp = make_channel(rlx, "fp64")
+    J, J0, adj, fd, adjA, fdA = _fd_checks(s, s, prof, amp, steps=60, interval=16, ncells=16, seed=7)
+    assert abs(J - J0) <= 1e-13 * abs(J0)
+    scale = np.max(np.abs(fd))
+    assert scale > 0
+    err = np.max(np.abs(adj - fd)) / scale
+    assert err < 1e-6, (err, adj, fd)
+    assert abs(adjA - fdA) <= 1e-6 * abs(fdA), (adjA, fdA)
+
+
+@pytest.mark.parametrize("rlx", [BGK, TRT, REGULARIZATION], ids=["bgk", "trt", "reg"])
+def test_adjoint_fp32_within_1e4_of_fp64_fd(rlx):
+    """FP32 forward/adjoint against FP64 central differences: the north-star
+    bar of 1e-4 relative (relative to the largest sampled gradient)."""
+    print(f"adjoint fp32 rlx={rlx}", flush=True)
+    s32, prof, amp = make_channel(rlx, "fp32")
+    s64, _, _ = make_channel(rlx, "fp64")
+    J, J0, adj, fd, adjA, fdA = _fd_checks(s32, s64, prof, amp, steps=60, interval=16, ncells=16, seed=11)
+    scale = np.max(np.abs(fd))
+    err = np.max(np.abs(adj - fd)) / scale
+    assert err < 1e-4, (err, adj, fd)
+    assert abs(adjA - fdA) <= 1e-4 * abs(fdA), (adjA, fdA)
+
+
+def test_adjoint_periodic_and_checkpoint_invariance():
+    """Gradients do not depend on the checkpoint interval; periodic faces
+    (halo-crossing sites in the special-site kernel) are covered."""
+    s, prof, amp = make_channel(TRT, "fp64", steps_bc=(PERIODIC, PERIODIC), seed=5)
+    acc, fden, fvel, g = _inputs(s)
+    s.engine.snapshot()
+    J1, ga1, gf1 = _adjoint(s, acc, fden, fvel, g, 37, 5)
+    J2, ga2, gf2 = _adjoint(s, acc, fden, fvel, g, 37, 37)
+    J3, ga3, gf3 = _adjoint(s, acc, fden, fvel, g, 37, 1)
+    assert J1 == J2 == J3
+    np.testing.assert_array_equal(ga1, ga2)
+    np.testing.assert_array_equal(ga1, ga3)
+    np.testing.assert_array_equal(gf1, gf2)
+    # and against finite differences at a few sites
+    J, J0, adj, fd, adjA, fdA = _fd_checks(s, s, prof, amp, steps=37, interval=5, ncells=8, seed=2)
+    assert np.max(np.abs(adj - fd)) <= 1e-6 * np.max(np.abs(fd))
+    assert abs(adjA - fdA) <= 1e-6 * abs(fdA)
+
+
+def test_adjoint_torch_autograd_function():
+    """The torch.autograd.Function returns the engine's gradients."""
+    import torch
+    from jyulb.adjoint import flux_objective
+    s, prof, amp = make_channel(BGK, "fp64", seed=9)
+    acc, fden, fvel, g = _inputs(s)
+    s.engine.snapshot()
+    J, ga, gf = _adjoint(s, acc, fden, fvel, g, 40, 8)
+    s.engine.restore()
+    acc_t = torch.tensor(acc, device="cuda", requires_grad=True)
+    fv_t = torch.tensor(fvel, device="cuda", requires_grad=True)
+    Jt = flux_objective(acc_t, fv_t, s, 40, 8, FACE_X1, X, FACE_1X)
+    Jt.backward()
+    assert abs(float(Jt) - J) <= 1e-14 * abs(J)
+    np.testing.assert_allclose(acc_t.grad.cpu().numpy(), ga, rtol=0, atol=1e-14 * np.max(np.abs(ga)))
+    np.testing.assert_allclose(fv_t.grad.cpu().numpy(), gf, rtol=0, atol=1e-14 * np.max(np.abs(gf)))

@@ -159,3 +159,35 @@ def test_adjoint_torch_autograd_function():
     assert abs(float(Jt) - J) <= 1e-14 * abs(J)
     np.testing.assert_allclose(acc_t.grad.cpu().numpy(), ga, rtol=0, atol=1e-14 * np.max(np.abs(ga)))
     np.testing.assert_allclose(fv_t.grad.cpu().numpy(), gf, rtol=0, atol=1e-14 * np.max(np.abs(gf)))
+
+
+@pytest.mark.parametrize("rlx", [BGK, TRT, REGULARIZATION], ids=["bgk", "trt", "reg"])
+def test_adjoint_matches_torch_fp64_autograd(rlx):
+    """Gradients of the engine's FP64 adjoint against torch.autograd through
+    an independent FP64 PyTorch restatement of the forward model (tests/torch_lb.py)
+    started from the same post-collision state."""
+    import torch
+    import torch_lb
+    print(f"torch reference rlx={rlx}", flush=True)
+    lat = (10, 6, 5)
+    s, prof, amp = make_channel(rlx, "fp64", lat=lat, steps_bc=(WALL, SYMMETRIC), seed=21)
+    acc, fden, fvel, g = _inputs(s)
+    s.engine.snapshot()
+    f0 = s.post_collision_f()[:, 1:-1, 1:-1, 1:-1].reshape(19, -1)
+    steps = 30
+    J, ga, gf = _adjoint(s, acc, fden, fvel, g, steps, 7)
+    ref = torch_lb.TorchLB(np.asarray(s.get_phase()).copy(), s.bc, rlx, 0.2 / 3.0)
+    acc_t = torch.tensor(acc.reshape(-1, 3), dtype=torch.float64, requires_grad=True)
+    v_t = torch.tensor(fvel.reshape(-1, 3), dtype=torch.float64, requires_grad=True)
+    rho, us = ref.run(torch.tensor(f0), acc_t, v_t, 1.0, torch.tensor(g), steps)
+    nx = lat[0]
+    out = torch.tensor(np.argwhere(np.asarray(s.get_phase()) != 0)[:, 0] == nx - 1)
+    Jt = (rho[out] * us[0][out]).mean()
+    Jt.backward()
+    assert abs(float(Jt) - J) <= 1e-12 * abs(J), (float(Jt), J)
+    gt = acc_t.grad.numpy().reshape(ga.shape)
+    fluid = np.asarray(s.get_phase()) != 0
+    scale = np.max(np.abs(gt[fluid]))
+    assert np.max(np.abs(ga[fluid] - gt[fluid])) <= 1e-9 * scale
+    gvt = v_t.grad.numpy().reshape(gf.shape)
+    np.testing.assert_allclose(gf[1:-1, 1:-1], gvt[1:-1, 1:-1], rtol=0, atol=1e-9 * np.max(np.abs(gvt)))

@@ -5,9 +5,13 @@ D3Q19 TRT (magic parameter 3/16), tau = 0.6, rho0 = 1, bounce-back walls on
 all faces, the lid a SOLID layer at the top z plane moving with ux = 0.05.
 A "step" is one fused AA stream+collide pass over the lattice, with the
 populations resident in HBM. The same workload runs in FP64 (the reference's
-numerics), FP32 on shifted populations and FP16 storage / FP32 arithmetic;
-the reduced-precision velocity fields are compared with the FP64 one after
-the same number of steps.
+numerics), FP32 on shifted populations, FP16 storage / FP32 arithmetic and
+FP32 storage / FP64 arithmetic; the reduced-precision velocity fields are
+compared with the FP64 one after the same number of steps. Beside it: the
+precision workload (Zou-He duct 512x128x128, regularized, 5000 steps) in each
+precision with its velocity error against FP64, and the adjoint workload
+(porous channel 128x64x64, 1000 steps, checkpoint every 50) with the
+adjoint/forward step-time ratio.
 
 Prints one JSON line (see --help). The CPU baseline is the FP64 CPU oracle
 (a C/OpenMP restatement of the reference algorithm) on a bounded sample of the
@@ -26,8 +30,8 @@ sys.path.insert(0, os.path.join(ROOT, "jyu-lb_amd"))
 
 METRIC = "MLUPS D3Q19 AA fp64 vs fp32-shifted; adjoint/forward step time ratio"
 PEAK_HBM_GBS = 8000.0
-BYTES_PER_SITE = {"fp64": 2 * 19 * 8, "fp32": 2 * 19 * 4, "fp16": 2 * 19 * 2}
-KERNEL = {"fp64": "k_step<0,1,*>", "fp32": "k_step<1,1,*>", "fp16": "k_step<2,1,*>"}
+BYTES_PER_SITE = {"fp64": 2 * 19 * 8, "fp32": 2 * 19 * 4, "fp16": 2 * 19 * 2, "fp32_storage": 2 * 19 * 4}
+KERNEL = {"fp64": "k_step<0,1,*>", "fp32": "k_step<1,1,*>", "fp16": "k_step<2,1,*>", "fp32_storage": "k_step<3,1,*>"}
 
 
 def cavity_inputs(n):
@@ -58,6 +62,9 @@ def time_device(solver, steps, warmup):
     import torch
     eng = solver.engine
     eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    # what Solver.evolve does on entry: Zou-He face data, gravity, acc, walls
+    solver._set_bc_data()
+    solver._sync_inputs()
     eng.evolve(warmup)
     eng.synchronize()
     torch.cuda.synchronize()
@@ -70,6 +77,113 @@ def time_device(solver, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     return t0.elapsed_time(t1) / 1e3, wall
+
+
+def duct(precision, lat=(512, 128, 128), peak=0.04, tau=0.55):
+    """Precision workload: Zou-He duct, regularized collision, tau = 0.55,
+    parabolic inlet (peak 0.04) at 1X, outlet density 1 at X1, bounce-back
+    walls in y and z."""
+    from jyulb.isothermal import Solver
+    from jyulb.defs import REGULARIZATION, WALL, FIXED_DEN, FIXED_VEL
+    nx, ny, nz = lat
+    s = Solver(np.array(lat, dtype=np.int32), REGULARIZATION, (tau - 0.5) / 3.0, np.zeros(3),
+               np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, WALL, WALL], dtype=np.int32), precision=precision)
+    s.get_phase()[...] = 1
+    ey = (np.arange(ny) + 0.5) / ny - 0.5
+    ez = (np.arange(nz) + 0.5) / nz - 0.5
+    s.get_vel()[0, :, :, 0] = peak * (1 - (2 * ey[:, None]) ** 2) * (1 - (2 * ez[None, :]) ** 2)
+    nfl = s.finalize_init()
+    return s, nfl
+
+
+def duct_precision(steps, precs):
+    """The duct in each precision over `steps` steps: MLUPS (HIP events over
+    the whole run, one evolve call) and the velocity error against FP64."""
+    out, fields, fl = {}, {}, None
+    for prec in precs:
+        s, nfl = duct(prec)
+        print(f"[bench] duct {prec}: {steps} steps", file=sys.stderr, flush=True)
+        t, _ = time_device(s, steps, 0)
+        fields[prec] = np.array(s.engine.download_den_vel()[1])
+        out[prec] = dict(mlups=round(nfl * steps / t / 1e6, 1))
+        del s
+    ref = fields.get("fp64")
+    if ref is not None:
+        r = ref.reshape(-1, 3)
+        for prec in precs:
+            if prec == "fp64":
+                continue
+            d = fields[prec].reshape(-1, 3) - r
+            out[prec]["vel_rel_l2"] = float(np.sqrt(np.sum(d * d) / np.sum(r * r)))
+            out[prec]["vel_rel_linf"] = float(np.max(np.abs(d)) / np.max(np.abs(r)))
+    return dict(workload="Zou-He duct 512x128x128, regularized, tau=0.55, inlet parabola peak 0.04, "
+                         "outlet rho=1, bounce-back side walls", steps=steps, variants=out)
+
+
+def porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02):
+    """Adjoint workload: porous channel of seeded spheres (radius 4..8, 10 %
+    solid, none in 8-layer buffers behind the inlet and outlet faces), BGK
+    tau = 0.7, gravity gx = 1e-5, parabolic inlet velocity face (peak 0.02) at
+    1X, outlet density 1 at X1, bounce-back walls in y and z."""
+    from jyulb.isothermal import Solver
+    from jyulb.geometry import porous_spheres
+    from jyulb.defs import BGK, WALL, FIXED_DEN, FIXED_VEL, FACE_1X, FACE_X1
+    nx, ny, nz = lat
+    s = Solver(np.array(lat, dtype=np.int32), BGK, 0.2 / 3.0, np.array([1e-5, 0.0, 0.0]),
+               np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, WALL, WALL], dtype=np.int32), precision=precision)
+    solid = porous_spheres(lat, 4, 8, 0.10, seed, open_faces=(FACE_1X, FACE_X1), open_depth=8)
+    s.get_phase()[...] = np.where(solid, 0, 1)
+    ey = (np.arange(ny) + 0.5) / ny - 0.5
+    ez = (np.arange(nz) + 0.5) / nz - 0.5
+    s.get_vel()[0, :, :, 0] = peak * (1 - (2 * ey[:, None]) ** 2) * (1 - (2 * ez[None, :]) ** 2)
+    nfl = s.finalize_init()
+    return s, nfl
+
+
+def time_adjoint(precision, steps, interval):
+    """Forward evolve vs the adjoint (forward pass with checkpoints, then the
+    reverse pass with per-segment recomputation) over the same steps, all on
+    torch's current stream, timed with HIP events."""
+    import torch
+    from jyulb.defs import FACE_1X, FACE_X1, X
+    from jyulb.adjoint import _prepare
+    s, nfl = porous_channel(precision)
+    eng = s.engine
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    _prepare(s)
+    eng.snapshot()
+    ga = torch.empty(tuple(s.get_acc().shape), dtype=torch.float64, device="cuda")
+    gf = torch.empty(eng.face_shape(FACE_1X) + (3,), dtype=torch.float64, device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def run(timed):
+        eng.restore()
+        torch.cuda.synchronize()
+        ev[0].record()
+        eng.evolve(steps)
+        ev[1].record()
+        eng.restore()
+        J = eng.adj_forward(steps, interval, FACE_X1, X)
+        ev[2].record()
+        eng.adj_backward_device(FACE_1X, ga.data_ptr(), gf.data_ptr())
+        ev[3].record()
+        torch.cuda.synchronize()
+        return J
+
+    run(False)  # allocates checkpoints and adjoint buffers
+    J = run(True)
+    t_fwd = ev[0].elapsed_time(ev[1]) / 1e3
+    t_af = ev[1].elapsed_time(ev[2]) / 1e3
+    t_ab = ev[2].elapsed_time(ev[3]) / 1e3
+    del s
+    return dict(workload="porous channel 128x64x64 (spheres r 4..8, 10% solid, 8-layer open buffers at inlet/outlet), BGK tau=0.7, gx=1e-5, "
+                         "Zou-He inlet (peak 0.02) / outlet; J = mean outlet x-flux",
+                precision=precision, steps=steps, checkpoint_interval=interval, fluid_sites=nfl, J=J,
+                forward_ms_per_step=round(1e3 * t_fwd / steps, 5),
+                adjoint_forward_ms_per_step=round(1e3 * t_af / steps, 5),
+                adjoint_reverse_ms_per_step=round(1e3 * t_ab / steps, 5),
+                adjoint_over_forward=round(t_ab / t_fwd, 3),
+                gradient_over_forward=round((t_af + t_ab) / t_fwd, 3))
 
 
 def cpu_baseline(n, steps):
@@ -111,10 +225,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256, help="cavity edge length")
-    ap.add_argument("--precisions", default="fp64,fp32,fp16")
+    ap.add_argument("--precisions", default="fp64,fp32,fp16,fp32_storage")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--layout", type=int, default=0, help="population layout (LB_LAYOUT_*)")
+    ap.add_argument("--adj-steps", type=int, default=1000, help="steps of the adjoint workload (0: skip)")
+    ap.add_argument("--duct-steps", type=int, default=5000, help="steps of the precision workload (0: skip)")
+    ap.add_argument("--adj-interval", type=int, default=50)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain device copy of 2 GiB (torch copy_), the reference "
                          "kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE counters")
@@ -168,6 +285,14 @@ def main():
             errors[p] = dict(vel_rel_l2=float(np.sqrt(np.sum(d * d)) / nref),
                              vel_linf=float(np.max(np.abs(d))),
                              vel_rel_linf=float(np.max(np.abs(d)) / np.max(np.abs(r))))
+    duct_res = duct_precision(args.duct_steps, precs) if args.duct_steps > 0 else None
+    adj = None
+    if args.adj_steps > 0:
+        print(f"[bench] adjoint: {args.adj_steps} steps, checkpoint every {args.adj_interval}",
+              file=sys.stderr, flush=True)
+        adj = time_adjoint("fp32", args.adj_steps, args.adj_interval)
+        print(f"[bench] adjoint/forward {adj['adjoint_over_forward']} (reverse pass), "
+              f"{adj['gradient_over_forward']} (whole gradient)", file=sys.stderr, flush=True)
     main_p = "fp64" if "fp64" in res else precs[0]
     m = res[main_p]
     cpu = None
@@ -197,7 +322,9 @@ def main():
         "variants": {p: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                      for p, r in res.items()},
         "precision_error_vs_fp64": errors,
+        "precision_workload": duct_res,
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = round(res["fp32"]["mlups"] / res["fp64"]["mlups"], 3)

@@ -39,6 +39,7 @@ extern "C" {
 #define LB_FP64 0          /* f64 populations, f64 arithmetic: the reference's numerics */
 #define LB_FP32_SHIFTED 1  /* f32 storage of f - w*rho0, f32 arithmetic */
 #define LB_FP16_STORAGE 2  /* f16 storage of s*(f - w*rho0), f32 arithmetic */
+#define LB_FP32_STORAGE 3  /* f32 storage of f - w*rho0, f64 arithmetic */
 
 /* Population layouts in device memory */
 #define LB_LAYOUT_ROWS 0        /* f[i][j][l][k]: the 19 populations of a lattice row side by side */
@@ -53,7 +54,7 @@ typedef struct {
     double kvisc;        /* kinematic viscosity; tau = 0.5 + 3 kvisc (isothermal.pyx:273-287) */
     double g[3];         /* gravity */
     int32_t bc[6];       /* per face, WALL=0 PERIODIC=1 FIXED_DEN=2 FIXED_VEL=3 SYMMETRIC=4 */
-    int32_t precision;   /* LB_FP64 / LB_FP32_SHIFTED / LB_FP16_STORAGE */
+    int32_t precision;   /* LB_FP64 / LB_FP32_SHIFTED / LB_FP16_STORAGE / LB_FP32_STORAGE */
     double rho0;         /* reference density of the shifted variants (1.0 if 0) */
     double fp16_scale;   /* storage scale s of LB_FP16_STORAGE (0: chosen at lb_finalize) */
     int32_t layout;      /* LB_LAYOUT_* */

@@ -69,6 +69,7 @@ template <int P> struct Var;
 template <> struct Var<LB_FP64> { using TS = double; using TC = double; static constexpr bool SH = false; };
 template <> struct Var<LB_FP32_SHIFTED> { using TS = float; using TC = float; static constexpr bool SH = true; };
 template <> struct Var<LB_FP16_STORAGE> { using TS = _Float16; using TC = float; static constexpr bool SH = true; };
+template <> struct Var<LB_FP32_STORAGE> { using TS = float; using TC = double; static constexpr bool SH = true; };
 
 template <class TC> LBD TC ldp(const double* p) { return (TC)(*p); }
 
@@ -105,7 +106,7 @@ LBD void stf(typename Var<P>::TS* p, typename Var<P>::TC v, float scale) {
         st_stream<NT>(p, (_Float16)(v * scale));
     else {
         (void)scale;
-        st_stream<NT>(p, v);
+        st_stream<NT>(p, (typename Var<P>::TS)v);
     }
 }
 
@@ -955,8 +956,8 @@ struct lb_solver {
 
 static constexpr int RED_PARTS = 1024;
 
-static size_t ts_size(int prec) { return prec == LB_FP64 ? 8 : prec == LB_FP32_SHIFTED ? 4 : 2; }
-static size_t tc_size(int prec) { return prec == LB_FP64 ? 8 : 4; }
+static size_t ts_size(int prec) { return prec == LB_FP64 ? 8 : prec == LB_FP16_STORAGE ? 2 : 4; }
+static size_t tc_size(int prec) { return prec == LB_FP64 || prec == LB_FP32_STORAGE ? 8 : 4; }
 
 static int dalloc(lb_solver* s, void** p, size_t n) {
     hipError_t e = hipMalloc(p, n);
@@ -1024,7 +1025,7 @@ static int check_config(const lb_config* c) {
     }
     if ((fx[0] && (fx[1] || fx[2])) || (fx[1] && (fx[0] || fx[2])) || (fx[2] && (fx[0] || fx[1])))
         return fail(LB_EINVAL, "Invalid fixed variable bc setup.");
-    if (c->precision < LB_FP64 || c->precision > LB_FP16_STORAGE) return fail(LB_EINVAL, "Invalid precision.");
+    if (c->precision < LB_FP64 || c->precision > LB_FP32_STORAGE) return fail(LB_EINVAL, "Invalid precision.");
     if (c->layout < LB_LAYOUT_ROWS || c->layout > LB_LAYOUT_SOA_PADDED) return fail(LB_EINVAL, "Invalid layout.");
     const int64_t nzp = (c->lat[2] + 63) / 64 * 64;
     const int64_t nq = (int64_t)c->lat[0] * c->lat[1] * nzp;
@@ -1042,7 +1043,7 @@ static int import_acc_from_stage(lb_solver* s) {
         int rc = dalloc(s, &s->acc, (size_t)3 * s->nq * tc_size(s->prec));
         if (rc) return rc;
     }
-    if (s->prec == LB_FP64)
+    if (tc_size(s->prec) == 8)
         k_import_acc<double><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, (double*)s->acc, s->d, s->nq);
     else
         k_import_acc<float><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, (float*)s->acc, s->d, s->nq);
@@ -1169,6 +1170,7 @@ static void launch_any_step(const lb_solver* s, const StepParams& p, bool odd) {
     switch (s->prec) {
     case LB_FP64: launch_step_rlx<LB_FP64>(p, s->cfg.rlx_scheme, odd, s->stream); break;
     case LB_FP32_SHIFTED: launch_step_rlx<LB_FP32_SHIFTED>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+    case LB_FP32_STORAGE: launch_step_rlx<LB_FP32_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
     default: launch_step_rlx<LB_FP16_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
     }
 }
@@ -1408,6 +1410,7 @@ int lb_finalize(lb_solver* s, int64_t* nfluid) {
     switch (s->prec) {
     case LB_FP64: launch_init<LB_FP64>(s); break;
     case LB_FP32_SHIFTED: launch_init<LB_FP32_SHIFTED>(s); break;
+    case LB_FP32_STORAGE: launch_init<LB_FP32_STORAGE>(s); break;
     default: launch_init<LB_FP16_STORAGE>(s); break;
     }
     HIPCHK(hipGetLastError());
@@ -1442,6 +1445,7 @@ int lb_evolve(lb_solver* s, uint32_t steps) {
         switch (s->prec) {
         case LB_FP64: launch_step_rlx<LB_FP64>(p, s->cfg.rlx_scheme, odd, s->stream); break;
         case LB_FP32_SHIFTED: launch_step_rlx<LB_FP32_SHIFTED>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+        case LB_FP32_STORAGE: launch_step_rlx<LB_FP32_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
         default: launch_step_rlx<LB_FP16_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
         }
         s->next_odd ^= 1;
@@ -1485,6 +1489,9 @@ int lb_download_post_f(lb_solver* s, double* f) {
     case LB_FP64: k_post_f<LB_FP64><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf); break;
     case LB_FP32_SHIFTED:
         k_post_f<LB_FP32_SHIFTED><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
+        break;
+    case LB_FP32_STORAGE:
+        k_post_f<LB_FP32_STORAGE><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
         break;
     default:
         k_post_f<LB_FP16_STORAGE><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
@@ -1558,7 +1565,8 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
     if (!s || !J || face < 0 || face > 5 || axis < 0 || axis > 2) return fail(LB_EINVAL, "invalid argument");
     if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
     if (steps < 1 || interval < 1) return fail(LB_EINVAL, "steps and checkpoint interval must be >= 1");
-    if (s->prec == LB_FP16_STORAGE) return fail(LB_EINVAL, "the adjoint runs in FP64 or FP32 (not FP16 storage)");
+    if (s->prec != LB_FP64 && s->prec != LB_FP32_SHIFTED)
+        return fail(LB_EINVAL, "the adjoint runs in FP64 or FP32-shifted precision");
     if (s->moving) return fail(LB_EINVAL, "the adjoint supports stationary walls only (no moving solid sites)");
     const uint32_t nseg = (steps + interval - 1) / interval;
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);

@@ -11,11 +11,12 @@ import numpy as np
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libjyulb_hip.so")
 
 LB_OK, LB_EINVAL, LB_ESTATE, LB_EHIP, LB_ENOMEM = 0, 1, 2, 3, 4
-LB_FP64, LB_FP32_SHIFTED, LB_FP16_STORAGE = 0, 1, 2
+LB_FP64, LB_FP32_SHIFTED, LB_FP16_STORAGE, LB_FP32_STORAGE = 0, 1, 2, 3
 
 PRECISIONS = {"fp64": LB_FP64, "fp32": LB_FP32_SHIFTED, "fp32_shifted": LB_FP32_SHIFTED,
-              "fp16": LB_FP16_STORAGE, "fp16_storage": LB_FP16_STORAGE,
-              LB_FP64: LB_FP64, LB_FP32_SHIFTED: LB_FP32_SHIFTED, LB_FP16_STORAGE: LB_FP16_STORAGE}
+              "fp16": LB_FP16_STORAGE, "fp16_storage": LB_FP16_STORAGE, "fp32_storage": LB_FP32_STORAGE,
+              LB_FP64: LB_FP64, LB_FP32_SHIFTED: LB_FP32_SHIFTED, LB_FP16_STORAGE: LB_FP16_STORAGE,
+              LB_FP32_STORAGE: LB_FP32_STORAGE}
 
 
 class LbConfig(ctypes.Structure):
@@ -42,6 +43,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} not found: build it with `python jyu-lb_amd/build.py`")
+    # One HIP runtime per process: torch ships its own libamdhip64 with the
+    # same soname as ROCm's. Loaded first, it is the one the engine binds to,
+    # so torch streams and device pointers are valid handles here; loaded
+    # after the engine, a second runtime would start and torch would see no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     H = ctypes.c_void_p
     dp = ctypes.POINTER(ctypes.c_double)

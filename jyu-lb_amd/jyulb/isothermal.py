@@ -14,8 +14,10 @@ All quantities are in lattice units (isothermal.pyx:1-36).
 
 Additions, keyword-only so that reference call sites are unaffected:
   precision  "fp64" (default; the reference's numerics), "fp32" (FP32
-             arithmetic on shifted populations f - w*rho0) or "fp16" (FP16
-             storage of the scaled shifted populations, FP32 arithmetic)
+             arithmetic on shifted populations f - w*rho0), "fp16" (FP16
+             storage of the scaled shifted populations, FP32 arithmetic) or
+             "fp32_storage" (FP32 storage of the shifted populations, FP64
+             arithmetic: the FP32 traffic with the rounding of storage only)
 """
 import numpy as np
 

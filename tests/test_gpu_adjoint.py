@@ -191,3 +191,51 @@ def test_adjoint_matches_torch_fp64_autograd(rlx):
     assert np.max(np.abs(ga[fluid] - gt[fluid])) <= 1e-9 * scale
     gvt = v_t.grad.numpy().reshape(gf.shape)
     np.testing.assert_allclose(gf[1:-1, 1:-1], gvt[1:-1, 1:-1], rtol=0, atol=1e-9 * np.max(np.abs(gvt)))
+
+
+def make_porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02):
+    """Adjoint workload: porous channel of seeded spheres (radius 4..8, 10 %
+    solid, none in 8-layer buffers behind the inlet and outlet faces), BGK
+    tau = 0.7, gravity gx = 1e-5, parabolic inlet (FIXED_VEL at 1X, peak
+    `peak`), outlet density 1 (FIXED_DEN at X1), walls in y and z."""
+    from jyulb.isothermal import Solver
+    from jyulb.geometry import porous_spheres
+    nx, ny, nz = lat
+    s = Solver(np.array(lat, dtype=np.int32), BGK, 0.2 / 3.0, np.array([1e-5, 0.0, 0.0]),
+               np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, WALL, WALL], dtype=np.int32), precision=precision)
+    solid = porous_spheres(lat, 4, 8, 0.10, seed, open_faces=(FACE_1X, FACE_X1), open_depth=8)
+    s.get_phase()[...] = np.where(solid, 0, 1)
+    prof = np.zeros((ny + 2, nz + 2))
+    ey = (np.arange(ny) + 0.5) / ny - 0.5
+    ez = (np.arange(nz) + 0.5) / nz - 0.5
+    prof[1:-1, 1:-1] = (1 - (2 * ey[:, None]) ** 2) * (1 - (2 * ez[None, :]) ** 2)
+    s.get_vel()[0, :, :, 0] = peak * prof[1:-1, 1:-1]
+    s.finalize_init()
+    return s, prof, peak
+
+
+def test_adjoint_porous_channel_full_size_fp32_vs_fp64_fd():
+    """128 x 64 x 64 porous channel, 1000 steps, checkpoint every 50: FP32
+    adjoint gradients of the mean outlet x-flux with respect to the body
+    force at 64 seeded random fluid cells and to the inlet amplitude, against
+    FP64 central differences (h = 1e-6). Bar: 1e-4 relative to the largest
+    sampled gradient (cells) and to the amplitude gradient."""
+    print("porous channel 128x64x64, 1000 steps, fp32 adjoint vs fp64 FD", flush=True)
+    s32, prof, amp = make_porous_channel("fp32")
+    s64, _, _ = make_porous_channel("fp64")
+    J, J0, adj, fd, adjA, fdA = _fd_checks(s32, s64, prof, amp, steps=1000, interval=50, ncells=64, seed=64)
+    scale = np.max(np.abs(fd))
+    err = np.max(np.abs(adj - fd)) / scale
+    print(f"J fp32 {J:.9e} fp64 {J0:.9e}; cell gradients max rel err {err:.2e}; "
+          f"amplitude {adjA:.9e} vs FD {fdA:.9e}", flush=True)
+    assert abs(J - J0) <= 1e-5 * abs(J0)
+    assert err < 1e-4, (err, adj, fd)
+    assert abs(adjA - fdA) <= 1e-4 * abs(fdA), (adjA, fdA)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "fp32_storage"])
+def test_adjoint_rejects_other_precisions(precision):
+    s, prof, amp = make_channel(BGK, precision)
+    acc, fden, fvel, g = _inputs(s)
+    with pytest.raises(ValueError, match="adjoint runs in FP64 or FP32"):
+        s.engine.adj_forward(10, 5, FACE_X1, X)

@@ -209,7 +209,7 @@ def cpu_baseline(n, steps):
 
 
 def load_traffic(prec):
-    path = os.path.join(ROOT, "profiles", "c2_cavity_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_step_cavity256_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -226,7 +226,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256, help="cavity edge length")
     ap.add_argument("--precisions", default="fp64,fp32,fp16,fp32_storage")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--layout", type=int, default=0, help="population layout (LB_LAYOUT_*)")
     ap.add_argument("--adj-steps", type=int, default=1000, help="steps of the adjoint workload (0: skip)")

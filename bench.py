@@ -140,6 +140,40 @@ def porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02):
     return s, nfl
 
 
+def porous_medium(precision, lat=(768, 384, 384), seed=5):
+    """Throughput workload: seeded random-sphere porous medium (radius 6..12,
+    20 % solid), TRT tau = 0.6, gravity gx = 1e-6, periodic on all faces
+    (113M lattice cells)."""
+    from jyulb.isothermal import Solver
+    from jyulb.geometry import porous_spheres
+    from jyulb.defs import TRT, PERIODIC
+    s = Solver(np.array(lat, dtype=np.int32), TRT, (0.6 - 0.5) / 3.0, np.array([1e-6, 0.0, 0.0]),
+               np.full(6, PERIODIC, dtype=np.int32), precision=precision)
+    solid = porous_spheres(lat, 6, 12, 0.20, seed)
+    s.get_phase()[...] = np.where(solid, 0, 1)
+    del solid
+    nfl = s.finalize_init()
+    return s, nfl
+
+
+def throughput_workload(steps, precs):
+    """The 768x384x384 porous medium in each precision: MLUPS over the fluid
+    sites and over all lattice cells (HIP events around one evolve call)."""
+    out = {}
+    ncell = 768 * 384 * 384
+    for prec in precs:
+        s, nfl = porous_medium(prec)
+        print(f"[bench] porous 768x384x384 {prec}: {nfl} fluid sites, {steps} steps", file=sys.stderr, flush=True)
+        t, _ = time_device(s, steps, 10)
+        out[prec] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
+                         ms_per_step=round(1e3 * t / steps, 4),
+                         achieved_gbs=round(BYTES_PER_SITE[prec] * nfl * steps / t / 1e9, 1),
+                         device_bytes=int(s.engine.info()["device_bytes"]))
+        del s
+    return dict(workload="porous medium 768x384x384 (spheres r 6..12, 20% solid), TRT tau=0.6, gx=1e-6, "
+                         "periodic", steps=steps, lattice_cells=ncell, variants=out)
+
+
 def time_adjoint(precision, steps, interval):
     """Forward evolve vs the adjoint (forward pass with checkpoints, then the
     reverse pass with per-segment recomputation) over the same steps, all on
@@ -231,6 +265,8 @@ def main():
     ap.add_argument("--layout", type=int, default=0, help="population layout (LB_LAYOUT_*)")
     ap.add_argument("--adj-steps", type=int, default=1000, help="steps of the adjoint workload (0: skip)")
     ap.add_argument("--duct-steps", type=int, default=5000, help="steps of the precision workload (0: skip)")
+    ap.add_argument("--porous768-steps", type=int, default=0,
+                    help="steps of the 768x384x384 porous-medium throughput workload (0: skip, the default)")
     ap.add_argument("--adj-interval", type=int, default=50)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain device copy of 2 GiB (torch copy_), the reference "
@@ -286,6 +322,7 @@ def main():
                              vel_linf=float(np.max(np.abs(d))),
                              vel_rel_linf=float(np.max(np.abs(d)) / np.max(np.abs(r))))
     duct_res = duct_precision(args.duct_steps, precs) if args.duct_steps > 0 else None
+    big = throughput_workload(args.porous768_steps, precs) if args.porous768_steps > 0 else None
     adj = None
     if args.adj_steps > 0:
         print(f"[bench] adjoint: {args.adj_steps} steps, checkpoint every {args.adj_interval}",
@@ -323,6 +360,7 @@ def main():
                      for p, r in res.items()},
         "precision_error_vs_fp64": errors,
         "precision_workload": duct_res,
+        "throughput_workload": big,
         "cpu_baseline": cpu,
         "adjoint": adj,
     }

@@ -156,7 +156,7 @@ def test_adjoint_torch_autograd_function():
     fv_t = torch.tensor(fvel, device="cuda", requires_grad=True)
     Jt = flux_objective(acc_t, fv_t, s, 40, 8, FACE_X1, X, FACE_1X)
     Jt.backward()
-    assert abs(float(Jt) - J) <= 1e-14 * abs(J)
+    assert abs(float(Jt.detach()) - J) <= 1e-14 * abs(J)
     np.testing.assert_allclose(acc_t.grad.cpu().numpy(), ga, rtol=0, atol=1e-14 * np.max(np.abs(ga)))
     np.testing.assert_allclose(fv_t.grad.cpu().numpy(), gf, rtol=0, atol=1e-14 * np.max(np.abs(gf)))
 

@@ -170,8 +170,40 @@ def throughput_workload(steps, precs):
                          achieved_gbs=round(BYTES_PER_SITE[prec] * nfl * steps / t / 1e9, 1),
                          device_bytes=int(s.engine.info()["device_bytes"]))
         del s
+    adj = None
+    if "fp32" in precs:
+        # adjoint step cost at checkpoint interval 25 (50 steps: 3 checkpoints)
+        import torch
+        from jyulb.defs import FACE_X1, X
+        from jyulb.adjoint import _prepare
+        s, nfl = porous_medium("fp32")
+        eng = s.engine
+        eng.set_stream(torch.cuda.current_stream().cuda_stream)
+        _prepare(s)
+        eng.snapshot()
+        ga = torch.empty(tuple(s.get_acc().shape), dtype=torch.float64, device="cuda")
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        n_adj, interval = 50, 25
+        for _ in range(2):  # the first pass allocates the checkpoints and adjoint buffers
+            eng.restore()
+            torch.cuda.synchronize()
+            ev[0].record()
+            eng.evolve(n_adj)
+            ev[1].record()
+            eng.restore()
+            eng.adj_forward(n_adj, interval, FACE_X1, X)
+            ev[2].record()
+            eng.adj_backward_device(-1, ga.data_ptr(), 0)
+            ev[3].record()
+            torch.cuda.synchronize()
+        t_f, t_af, t_ab = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]))
+        adj = dict(precision="fp32", steps=n_adj, checkpoint_interval=interval,
+                   forward_ms_per_step=round(t_f / n_adj, 4), adjoint_reverse_ms_per_step=round(t_ab / n_adj, 4),
+                   adjoint_over_forward=round(t_ab / t_f, 3), gradient_over_forward=round((t_af + t_ab) / t_f, 3),
+                   device_bytes=int(eng.info()["device_bytes"]))
+        del s
     return dict(workload="porous medium 768x384x384 (spheres r 6..12, 20% solid), TRT tau=0.6, gx=1e-6, "
-                         "periodic", steps=steps, lattice_cells=ncell, variants=out)
+                         "periodic", steps=steps, lattice_cells=ncell, variants=out, adjoint=adj)
 
 
 def time_adjoint(precision, steps, interval):

@@ -196,6 +196,7 @@ struct StepParams {
     int64_t rs;  // element stride between rows (i,j) of the population array
     int64_t ps;  // element stride between populations
     const uint8_t* wclass;  // per wave of 64 sites: 0 = plain (see k_mask)
+    uint32_t xcd;           // deal tiles to XCDs in contiguous ranges (tile_index)
     uint32_t nwaves;
     const uint32_t* special;  // sites with M_SPECIAL, ascending within each wave
     uint32_t nspecial;
@@ -411,16 +412,18 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     }
 }
 
-#ifndef LB_XCD_REMAP
-#define LB_XCD_REMAP 0
+#ifndef LB_XCD_REMAP_CELLS
+#define LB_XCD_REMAP_CELLS (1u << 25)
 #endif
-// Tile of this workgroup. With LB_XCD_REMAP the tiles are dealt so that each
-// of the 8 XCDs (which receive workgroups round-robin) works through one
-// contiguous range of rows; the grid is padded to a multiple of 8. Off by
-// default: on the 256^3 cavity, the 512x128x128 duct and the 128x64x64 porous
-// channel the plain round-robin order measured 1.5-3.8% faster in FP64.
-LBD uint32_t tile_index() {
-    if (!LB_XCD_REMAP) return blockIdx.x;
+// Tile of this workgroup. With p.xcd the tiles are dealt so that each of the
+// 8 XCDs (which receive workgroups round-robin) works through one contiguous
+// range of rows; the grid is then padded to a multiple of 8. It is set for
+// lattices of at least LB_XCD_REMAP_CELLS cells (an environment variable of
+// that name, read at lb_create, overrides it): the 768x384x384 porous
+// medium ran 5-7% faster with it, while the 256^3 cavity, the 512x128x128
+// duct and the 128x64x64 porous channel ran 1.5-3.8% faster without.
+LBD uint32_t tile_index(const StepParams& p) {
+    if (!p.xcd) return blockIdx.x;
     const uint32_t per = gridDim.x >> 3;
     return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
 }
@@ -428,7 +431,7 @@ LBD uint32_t tile_index() {
 // Bulk kernel: every fluid site that is not special.
 template <int P, int RLX, bool ODD>
 __global__ __launch_bounds__(TILE) LB_OCC void k_step(StepParams p) {
-    const uint32_t tile = tile_index();
+    const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
     const uint32_t c = tile * TILE + threadIdx.x;
     // nq is a multiple of 64, so a wave is entirely inside or outside [0, nq)
@@ -472,23 +475,39 @@ struct AdjParams {
     double obj_scale;   // 1 / (fluid sites on the objective face)
 };
 
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
+// One site of the adjoint step. MODE_PLAIN / MODE_BULK address the
+// cotangents like the bulk step kernel (wave-uniform row bases, bounce-back
+// links from the link mask); MODE_SPECIAL resolves every link.
+template <int P, int RLX, bool ODD, int MODE>
+LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, const uint32_t m) {
     using TC = typename Var<P>::TC;
-    const uint32_t c = blockIdx.x * TILE + threadIdx.x;
-    if (c >= p.nq) return;
-    const uint32_t m = p.mask[c];
-    if (!(m & M_FLUID)) return;
+    constexpr bool SPECIAL = MODE == MODE_SPECIAL;
     const int64_t nq = p.nq;
-    const int64_t e = eidx(p, c);
-    int i, j, k;
-    decompose(c, p.d.ny, p.d.nzp, i, j, k);
     TC* L = (TC*)x.lam;
+    int i = 0, j = 0, k = 0;
+    if (SPECIAL || x.seed) decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    const int64_t e = eidx(p, c);
+    const uint32_t nzp = (uint32_t)p.d.nzp;
+    const uint32_t row = SPECIAL ? 0u : __builtin_amdgcn_readfirstlane(c / nzp);
+    const uint32_t kk = c - row * nzp;
+    TC* const Lr = L + (int64_t)row * p.rs;
+    auto own = [&](int l) { return uniform_ptr(Lr + (int64_t)l * p.ps); };
+    auto up = [&](int l) {
+        return uniform_ptr(Lr - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs + (int64_t)opp(l) * p.ps - CK[l]);
+    };
+    // location of the cotangent of f_l's forward read (= of f*_opp(l)'s write)
+    auto loc = [&](int l) -> TC* {
+        if constexpr (SPECIAL) return L + odd_loc<true>(p, m, e, l, i, j, k);
+        else if (l == REST || (MODE == MODE_BULK && (m & (1u << l)))) return own(l) + kk;
+        else return up(l) + kk;
+    };
+    constexpr bool NTE = (LB_NT_MASK & 1) != 0;
     TC lam[Q], mu[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        if (ODD) lam[opp(l)] = L[odd_loc<true>(p, m, e, l, i, j, k)];
-        else lam[l] = L[e + (int64_t)opp(l) * p.ps];
+        if constexpr (ODD) lam[opp(l)] = *loc(l);
+        else if constexpr (SPECIAL) lam[l] = L[e + (int64_t)opp(l) * p.ps];
+        else lam[l] = ld_stream<NTE>(own(opp(l)) + kk);
     }
     const TC* R = (const TC*)x.rec;
     const TC rho = R[c];
@@ -517,7 +536,7 @@ __global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) G[(int64_t)a * nq + c] += ga[a];
     }
-    if (m & M_ZH_ANY) {
+    if (SPECIAL && (m & M_ZH_ANY)) {
         for (int fc = 5; fc >= 0; --fc) {
             if (!(m & (M_ZH0 << fc))) continue;
             int64_t fi;
@@ -537,9 +556,36 @@ __global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
     }
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
-        if (ODD) L[odd_loc<true>(p, m, e, l, i, j, k)] = mu[l];
-        else L[e + (int64_t)l * p.ps] = mu[l];
+        if constexpr (ODD) *loc(l) = mu[l];
+        else if constexpr (SPECIAL) L[e + (int64_t)l * p.ps] = mu[l];
+        else st_stream<NTE>(own(l) + kk, mu[l]);
     }
+}
+
+// Adjoint step over the non-special fluid sites (same tiling as k_step).
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
+    const uint32_t tile = tile_index(p);
+    if (tile >= p.ntiles) return;
+    const uint32_t c = tile * TILE + threadIdx.x;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
+    if (w >= p.nwaves) return;
+    if (LB_PLAIN_PATH && p.wclass[w] == 0) {
+        adj_site<P, RLX, ODD, MODE_PLAIN>(p, x, c, M_FLUID);
+        return;
+    }
+    const uint32_t m = p.mask[c];
+    if ((m & (M_FLUID | M_SPECIAL)) != M_FLUID) return;
+    adj_site<P, RLX, ODD, MODE_BULK>(p, x, c, m);
+}
+
+// Adjoint step over the special sites (compact list).
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(TILE) void k_adj_special(StepParams p, AdjParams x) {
+    const uint32_t t = blockIdx.x * TILE + threadIdx.x;
+    if (t >= p.nspecial) return;
+    const uint32_t c = p.special[t];
+    adj_site<P, RLX, ODD, MODE_SPECIAL>(p, x, c, p.mask[c]);
 }
 
 // objective: sum over fluid sites of the face layer of rho u*_axis (FP64)
@@ -912,6 +958,7 @@ struct lb_solver {
     double rho0;
     double scale;
     hipStream_t stream = nullptr;
+    uint64_t xcd_cells = LB_XCD_REMAP_CELLS;  // tile order threshold (tile_index)
     void* f = nullptr;
     uint32_t* mask = nullptr;
     uint8_t* wclass = nullptr;
@@ -994,6 +1041,7 @@ static StepParams params(const lb_solver* s) {
     p.NK = s->NK;
     p.nq = s->nq;
     p.ntiles = (s->nq + TILE - 1) / TILE;
+    p.xcd = s->nq >= s->xcd_cells ? 1u : 0u;
     p.rs = s->rs;
     p.ps = s->ps;
     p.wclass = s->wclass;
@@ -1151,7 +1199,7 @@ static void launch_init(lb_solver* s) {
 
 template <int P, int RLX>
 static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
-    const unsigned grid = LB_XCD_REMAP ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
     if (odd) k_step<P, RLX, true><<<grid, TILE, 0, st>>>(p);
     else k_step<P, RLX, false><<<grid, TILE, 0, st>>>(p);
     if (p.nspecial) {
@@ -1179,9 +1227,14 @@ static void launch_any_step(const lb_solver* s, const StepParams& p, bool odd) {
 
 template <int P, int RLX>
 static void launch_adj(const lb_solver* s, const StepParams& p, const AdjParams& x, bool odd) {
-    const unsigned grid = (s->nq + TILE - 1) / TILE;
+    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
     if (odd) k_adj<P, RLX, true><<<grid, TILE, 0, s->stream>>>(p, x);
     else k_adj<P, RLX, false><<<grid, TILE, 0, s->stream>>>(p, x);
+    if (p.nspecial) {
+        const unsigned g2 = (p.nspecial + TILE - 1) / TILE;
+        if (odd) k_adj_special<P, RLX, true><<<g2, TILE, 0, s->stream>>>(p, x);
+        else k_adj_special<P, RLX, false><<<g2, TILE, 0, s->stream>>>(p, x);
+    }
 }
 
 template <int P>
@@ -1223,6 +1276,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     s->nq = (uint32_t)((int64_t)s->d.nx * s->d.ny * s->d.nzp);
     s->ncell = (int64_t)s->d.nx * s->d.ny * s->d.nz;
     s->prec = cfg->precision;
+    if (const char* env = getenv("LB_XCD_REMAP_CELLS")) s->xcd_cells = strtoull(env, nullptr, 10);
     s->rho0 = cfg->rho0 > 0 ? cfg->rho0 : 1.0;
     s->scale = cfg->fp16_scale > 0 ? cfg->fp16_scale : 1.0;
     // relaxation parameters, isothermal.pyx:273-287

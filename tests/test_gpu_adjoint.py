@@ -239,3 +239,19 @@ def test_adjoint_rejects_other_precisions(precision):
     acc, fden, fvel, g = _inputs(s)
     with pytest.raises(ValueError, match="adjoint runs in FP64 or FP32"):
         s.engine.adj_forward(10, 5, FACE_X1, X)
+
+
+def test_adjoint_identical_with_xcd_tile_order(monkeypatch):
+    """The XCD tile order (used on large lattices) does not change a bit of
+    the forward objective or the gradients."""
+    out = []
+    for env in (None, "0"):
+        if env is not None:
+            monkeypatch.setenv("LB_XCD_REMAP_CELLS", env)
+        s, prof, amp = make_channel(TRT, "fp32", steps_bc=(PERIODIC, PERIODIC), seed=8)
+        acc, fden, fvel, g = _inputs(s)
+        s.engine.snapshot()
+        out.append(_adjoint(s, acc, fden, fvel, g, 30, 7))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])

@@ -112,3 +112,12 @@ def test_gpu_fp64_bit_exact(path, layout):
         if k.startswith("f_"):
             a, b = _fluid(a, z["sites"]), _fluid(b, z["sites"])
         np.testing.assert_array_equal(b, a, err_msg=f"{sc['name']} {k} layout {layout}")
+
+
+@pytest.mark.parametrize("name", ["trt_periodic", "bgk_zouhe_x", "reg_symwall"])
+def test_gpu_fp64_bit_exact_with_xcd_tile_order(name, monkeypatch):
+    """Large lattices deal the step kernel's tiles to the XCDs in contiguous
+    ranges; forcing that order on small cases must not change a bit."""
+    monkeypatch.setenv("LB_XCD_REMAP_CELLS", "0")
+    path = os.path.join(os.path.dirname(__file__), "golden", name + ".npz")
+    test_gpu_fp64_bit_exact(path, 0)

@@ -197,6 +197,8 @@ struct StepParams {
     int64_t ps;  // element stride between populations
     const uint8_t* wclass;  // per wave of 64 sites: 0 = plain (see k_mask)
     uint32_t xcd;           // deal tiles to XCDs in contiguous ranges (tile_index)
+    const uint32_t* mov_idx;  // per site: index of a moving site in ladd_u
+    const double* ladd_u;     // per moving site: the 18 wall-velocity sums of Ladd's term
     uint32_t nwaves;
     const uint32_t* special;  // sites with M_SPECIAL, ascending within each wave
     uint32_t nspecial;
@@ -229,39 +231,48 @@ LBD int64_t odd_loc(const StepParams& p, uint32_t m, int64_t e, int l, int i, in
 
 // Ladd's moving-wall term for bounce-back link l (isothermal.pyx:1112-1248):
 // f_l = f_opp +/- cff*(u_a +/- u_b), u the velocity of the solid site
-// x - c_l (mirrored through symmetric faces, zero for WALL/FIXED halos).
-template <class T, int L>
-LBD void ladd(const StepParams& p, T* f, T cff1, T cff2, int i, int j, int k) {
+// x - c_l (mirrored through symmetric faces, zero for WALL/FIXED halos), a and
+// b the non-zero components of c_l, cff = 2 rho_prev w_l 3. The velocity sum
+// (u_a, or u_a +/- u_b for diagonal links) depends only on the walls, so it is
+// tabulated per moving site (k_ladd_prep, 18 links) whenever the walls change;
+// the step kernel adds cff * sum with the sign of c_l's first component.
+template <int L>
+LBD double ladd_sum(const Dims& d, const double* vel, int64_t nq, int i, int j, int k) {
     int yi = i - CI[L], yj = j - CJ[L], yk = k - CK[L], lp = L, fl = 0;
-    T u[3] = {T(0), T(0), T(0)};
-    if (resolve(p.d, yi, yj, yk, lp, fl)) {
-        const int64_t y = ((int64_t)yi * p.d.ny + yj) * p.d.nzp + yk;
+    double u[3] = {0.0, 0.0, 0.0};
+    if (resolve(d, yi, yj, yk, lp, fl)) {
+        const int64_t y = ((int64_t)yi * d.ny + yj) * d.nzp + yk;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            T v = (T)p.vel[(int64_t)a * p.nq + y];
+            double v = vel[(int64_t)a * nq + y];
             u[a] = (fl >> a & 1) ? -v : v;
         }
     }
     constexpr int c[3] = {CI[L], CJ[L], CK[L]};
     constexpr int a = c[0] != 0 ? 0 : (c[1] != 0 ? 1 : 2);
     constexpr int b = nnz(L) == 1 ? -1 : (a == 0 ? (c[1] != 0 ? 1 : 2) : 2);
-    T term;
-    if constexpr (b < 0) term = cff1 * u[a];
-    else if constexpr (c[a] == c[b]) term = cff2 * (u[a] + u[b]);
-    else term = cff2 * (u[a] - u[b]);
-    if constexpr (c[a] > 0) f[L] = f[L] + term;
-    else f[L] = f[L] - term;
+    if constexpr (b < 0) return u[a];
+    else if constexpr (c[a] == c[b]) return u[a] + u[b];
+    else return u[a] - u[b];
 }
 
-template <class T, int L>
-LBD void ladd_if(const StepParams& p, uint32_t m, T* f, T cff1, T cff2, int i, int j, int k) {
-    if (m & (1u << L)) ladd<T, L>(p, f, cff1, cff2, i, j, k);
-}
+// table slot of link l (the rest population has none)
+LBD constexpr int ladd_slot(int l) { return l < REST ? l : l - 1; }
+// sign of the first non-zero component of c_l
+LBD constexpr bool ladd_plus(int l) { return (CI[l] != 0 ? CI[l] : CJ[l] != 0 ? CJ[l] : CK[l]) > 0; }
 
-template <class T, int... Ls>
-LBD void ladd_all(const StepParams& p, uint32_t m, T* f, T cff1, T cff2, int i, int j, int k,
-                  std::integer_sequence<int, Ls...>) {
-    (ladd_if<T, Ls>(p, m, f, cff1, cff2, i, j, k), ...);
+template <class T>
+LBD void ladd_apply(const StepParams& p, uint32_t c, uint32_t m, T* f) {
+    const T den = (T)p.den[c];
+    const T cff1 = T(2.0) * den * T(2.0 / 36.0) * T(3.0);
+    const T cff2 = T(2.0) * den * T(1.0 / 36.0) * T(3.0);
+    const double* U = p.ladd_u + (int64_t)p.mov_idx[c] * (Q - 1);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        if (l == REST || !(m & (1u << l))) continue;
+        const T term = (nnz(l) == 1 ? cff1 : cff2) * (T)U[ladd_slot(l)];
+        f[l] = ladd_plus(l) ? f[l] + term : f[l] - term;
+    }
 }
 
 #ifndef LB_PLAIN_PATH
@@ -297,7 +308,7 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     const int64_t nq = p.nq;
     const int64_t e = eidx(p, c);
     int i = 0, j = 0, k = 0;
-    if (MODE == MODE_SPECIAL || (MODE == MODE_BULK && (m & M_MOVING))) decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    if (MODE == MODE_SPECIAL) decompose(c, p.d.ny, p.d.nzp, i, j, k);
 
     // Bulk sites address populations as a wave-uniform row base plus the
     // unsigned lane offset kk (a wave never straddles rows: nzp % 64 == 0), so
@@ -342,12 +353,7 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     const TC rho0 = (TC)p.rho0;
 
     // ---- Ladd moving walls on bounce-back links (previous-step density) ----
-    if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) {
-        const TC den = (TC)p.den[c];
-        const TC cff1 = TC(2.0) * den * TC(2.0 / 36.0) * TC(3.0);
-        const TC cff2 = TC(2.0) * den * TC(1.0 / 36.0) * TC(3.0);
-        ladd_all<TC>(p, m, f, cff1, cff2, i, j, k, std::make_integer_sequence<int, Q>{});
-    }
+    if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) ladd_apply<TC>(p, c, m, f);
 
     // ---- Zou-He faces, order 1X, X1, 1Y, Y1, 1Z, Z1 (isothermal.pyx:1280-1291) ----
     if (!PLAIN && (m & M_ZH_ANY)) {
@@ -959,6 +965,9 @@ struct lb_solver {
     double scale;
     hipStream_t stream = nullptr;
     uint64_t xcd_cells = LB_XCD_REMAP_CELLS;  // tile order threshold (tile_index)
+    uint32_t* mov_idx = nullptr;  // moving-site index per site (k_ladd_prep)
+    double* ladd_u = nullptr;     // 18 Ladd velocity sums per moving site
+    size_t ladd_cap = 0;          // moving sites ladd_u has room for
     void* f = nullptr;
     uint32_t* mask = nullptr;
     uint8_t* wclass = nullptr;
@@ -1050,6 +1059,8 @@ static StepParams params(const lb_solver* s) {
     p.nspecial = s->nspecial;
 
     p.moving = s->moving ? 1 : 0;
+    p.mov_idx = s->mov_idx;
+    p.ladd_u = s->ladd_u;
     return p;
 }
 
@@ -1143,6 +1154,39 @@ __global__ void k_list_special(const uint32_t* mask, uint32_t nq, uint32_t* list
     if (sp) list[base + __popcll(b & ((1ull << lane) - 1ull))] = c;
 }
 
+// Moving sites (fluid sites with a bounce-back link to a moving solid site):
+// an index per site and the 18 velocity sums of Ladd's term per site.
+template <int... Ls>
+LBD void ladd_sums(const Dims& d, const double* vel, int64_t nq, int i, int j, int k, uint32_t m, double* out,
+                   std::integer_sequence<int, Ls...>) {
+    ((out[Ls] = (Ls != REST && (m & (1u << Ls))) ? ladd_sum<Ls>(d, vel, nq, i, j, k) : 0.0), ...);
+}
+
+__global__ void k_ladd_prep(const uint32_t* mask, const double* vel, Dims d, uint32_t nq, uint32_t* mov_idx,
+                            double* ladd_u, unsigned int* counter) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;  // whole waves only: nq % 64 == 0
+    const uint32_t m = mask[c];
+    const bool mv = (m & M_FLUID) && (m & M_MOVING);
+    const unsigned long long b = __ballot(mv);
+    if (!b) return;
+    const int lane = threadIdx.x & 63;
+    unsigned int base = 0;
+    if (lane == 0) base = atomicAdd(counter, (unsigned int)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (!mv) return;
+    const uint32_t q = base + __popcll(b & ((1ull << lane) - 1ull));
+    mov_idx[c] = q;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    double* out = ladd_u + (int64_t)q * (Q - 1);
+    double tmp[Q];  // by link; the rest population has no table slot
+    ladd_sums(d, vel, nq, i, j, k, m, tmp, std::make_integer_sequence<int, Q>{});
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        if (l != REST) out[ladd_slot(l)] = tmp[l];
+}
+
 static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
     unsigned long long* d = (unsigned long long*)s->red;
     HIPCHK(hipMemsetAsync(d, 0, 24, s->stream));
@@ -1169,6 +1213,26 @@ static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
         unsigned int* cnt = (unsigned int*)s->red;
         HIPCHK(hipMemsetAsync(cnt, 0, 4, s->stream));
         k_list_special<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->nq, s->special, cnt);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    const size_t nmov = (size_t)*nmoving;
+    if (nmov > 0) {
+        int rc;
+        if (!s->mov_idx && (rc = dalloc(s, (void**)&s->mov_idx, (size_t)s->nq * 4))) return rc;
+        if (nmov > s->ladd_cap) {
+            if (s->ladd_u) {
+                HIPCHK(hipFree(s->ladd_u));
+                s->bytes -= s->ladd_cap * (Q - 1) * 8;
+                s->ladd_u = nullptr;
+            }
+            if ((rc = dalloc(s, (void**)&s->ladd_u, nmov * (Q - 1) * 8))) return rc;
+            s->ladd_cap = nmov;
+        }
+        unsigned int* cnt = (unsigned int*)s->red;
+        HIPCHK(hipMemsetAsync(cnt, 0, 4, s->stream));
+        k_ladd_prep<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->vel, s->d, s->nq, s->mov_idx,
+                                                                  s->ladd_u, cnt);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s->stream));
     }
@@ -1356,7 +1420,7 @@ int lb_destroy(lb_solver* s) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
                     s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
-                    s->snap,  s->snap_dv};
+                    s->snap,  s->snap_dv, s->mov_idx, s->ladd_u};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {

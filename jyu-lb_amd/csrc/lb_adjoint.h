@@ -120,10 +120,13 @@ LBD void collide_vjp(const T* lam, T rho, const T* us, const T* acc, const T* pr
 // kind FIXED_DEN: den = face density; FIXED_VEL: den = density produced by
 // the condition (the site's recorded density). Adds the cotangent of the face
 // velocity to gv[3].
-template <class T>
-LBD void zh_vjp(int face, int kind, T den, const T* vel, const T* mu, T* nu, T* gv) {
-    const int axis = face >> 1;
-    const int sgn_in = (face & 1) ? -1 : 1;  // component of the populations entering the domain
+// FACE is a template parameter so that the unit evaluations below fold to
+// constants at compile time; zh_vjp dispatches on the runtime face.
+template <class T, int FACE>
+LBD void zh_vjp_face(int kind, T den, const T* vel, const T* mu, T* nu, T* gv) {
+    constexpr int face = FACE;
+    constexpr int axis = face >> 1;
+    constexpr int sgn_in = (face & 1) ? -1 : 1;  // component of the populations entering the domain
     bool unk[Q];
 #pragma unroll
     for (int l = 0; l < Q; ++l) unk[l] = ccomp(l, axis) == sgn_in;
@@ -187,6 +190,18 @@ LBD void zh_vjp(int face, int kind, T den, const T* vel, const T* mu, T* nu, T* 
 #pragma unroll
         for (int a = 0; a < 3; ++a) gv[a] += cj[a] * den;
         gv[axis] += cv * den * T(sgn_in) / dn;
+    }
+}
+
+template <class T>
+LBD void zh_vjp(int face, int kind, T den, const T* vel, const T* mu, T* nu, T* gv) {
+    switch (face) {
+    case 0: zh_vjp_face<T, 0>(kind, den, vel, mu, nu, gv); break;
+    case 1: zh_vjp_face<T, 1>(kind, den, vel, mu, nu, gv); break;
+    case 2: zh_vjp_face<T, 2>(kind, den, vel, mu, nu, gv); break;
+    case 3: zh_vjp_face<T, 3>(kind, den, vel, mu, nu, gv); break;
+    case 4: zh_vjp_face<T, 4>(kind, den, vel, mu, nu, gv); break;
+    default: zh_vjp_face<T, 5>(kind, den, vel, mu, nu, gv); break;
     }
 }
 

@@ -197,6 +197,7 @@ struct StepParams {
     int64_t ps;  // element stride between populations
     const uint8_t* wclass;  // per wave of 64 sites: 0 = plain (see k_mask)
     uint32_t xcd;           // deal tiles to XCDs in contiguous ranges (tile_index)
+    uint32_t lds[2];        // dynamic LDS per workgroup of the bulk step launch (even, odd): caps occupancy
     const uint32_t* mov_idx;  // per site: index of a moving site in ladd_u
     const double* ladd_u;     // per moving site: the 18 wall-velocity sums of Ladd's term
     uint32_t nwaves;
@@ -222,7 +223,7 @@ template <bool SPECIAL = true>
 LBD int64_t odd_loc(const StepParams& p, uint32_t m, int64_t e, int l, int i, int j, int k) {
     if (l == REST) return e + (int64_t)REST * p.ps;
     if (m & (1u << l)) return e + (int64_t)l * p.ps;
-    if (!SPECIAL || !(m & M_BND))
+    if (!SPECIAL || !(m & (M_BND | M_KWRAP)))
         return e - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs - CK[l] + (int64_t)opp(l) * p.ps;
     int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
     resolve(p.d, yi, yj, yk, lp, fl);
@@ -322,6 +323,13 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     auto up = [&](int l) {
         return uniform_ptr(Fr - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs + (int64_t)opp(l) * p.ps - CK[l]);
     };
+    // z-periodic wrap of link l for this lane: the upstream element is the
+    // other end of the same upstream row (a uniform address)
+    const uint32_t nzm = (uint32_t)p.d.nz - 1u;
+    auto wraps = [&](int l) {
+        return MODE == MODE_BULK && (m & M_KWRAP) && ((CK[l] > 0 && kk == 0) || (CK[l] < 0 && kk == nzm));
+    };
+    auto upw = [&](int l) { return up(l) + CK[l] + (CK[l] > 0 ? nzm : 0u); };
     constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
     constexpr bool NTLO = (LB_NT_MASK & 4) != 0, NTSO = (LB_NT_MASK & 8) != 0;
 
@@ -338,8 +346,9 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
             constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTLO;
             if (l == REST) f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
             else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
-                if (CK[l] != 0) f[l] = ldf<P, NTLO>(up(l) + kk, p.inv_scale);
-                else f[l] = ldf<P, NTK>(up(l) + kk, p.inv_scale);
+                if (CK[l] == 0) f[l] = ldf<P, NTK>(up(l) + kk, p.inv_scale);
+                else if (wraps(l)) f[l] = ldf<P, NTLO>(upw(l), p.inv_scale);
+                else f[l] = ldf<P, NTLO>(up(l) + kk, p.inv_scale);
             } else f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
         }
     } else if constexpr (!ODD) {
@@ -405,8 +414,9 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
             constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTSO;
             if (l == REST) stf<P, NTK>(own(l) + kk, f[l], p.scale);
             else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
-                if (CK[l] != 0) stf<P, NTSO>(up(l) + kk, f[opp(l)], p.scale);
-                else stf<P, NTK>(up(l) + kk, f[opp(l)], p.scale);
+                if (CK[l] == 0) stf<P, NTK>(up(l) + kk, f[opp(l)], p.scale);
+                else if (wraps(l)) stf<P, NTSO>(upw(l), f[opp(l)], p.scale);
+                else stf<P, NTSO>(up(l) + kk, f[opp(l)], p.scale);
             } else stf<P, NTK>(own(l) + kk, f[opp(l)], p.scale);
         }
     } else if constexpr (!ODD) {
@@ -502,9 +512,12 @@ LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, con
         return uniform_ptr(Lr - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs + (int64_t)opp(l) * p.ps - CK[l]);
     };
     // location of the cotangent of f_l's forward read (= of f*_opp(l)'s write)
+    const uint32_t nzm = (uint32_t)p.d.nz - 1u;
     auto loc = [&](int l) -> TC* {
         if constexpr (SPECIAL) return L + odd_loc<true>(p, m, e, l, i, j, k);
         else if (l == REST || (MODE == MODE_BULK && (m & (1u << l)))) return own(l) + kk;
+        else if (MODE == MODE_BULK && (m & M_KWRAP) && ((CK[l] > 0 && kk == 0) || (CK[l] < 0 && kk == nzm)))
+            return up(l) + CK[l] + (CK[l] > 0 ? nzm : 0u);
         else return up(l) + kk;
     };
     constexpr bool NTE = (LB_NT_MASK & 1) != 0;
@@ -669,7 +682,8 @@ __global__ void k_mask(const uint8_t* phase, const double* vel, uint32_t* mask, 
         for (int l = 0; l < Q; ++l) {
             if (l == REST) continue;
             int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
-            const bool crossed = yi < 0 || yi >= d.nx || yj < 0 || yj >= d.ny || yk < 0 || yk >= d.nz;
+            const bool crossed_ij = yi < 0 || yi >= d.nx || yj < 0 || yj >= d.ny;
+            const bool crossed_k = yk < 0 || yk >= d.nz;
             if (!resolve(d, yi, yj, yk, lp, fl)) {
                 m |= 1u << l;
                 continue;
@@ -679,8 +693,10 @@ __global__ void k_mask(const uint8_t* phase, const double* vel, uint32_t* mask, 
                 m |= 1u << l;
                 if (vel && (vel[y] != 0.0 || vel[nq + y] != 0.0 || vel[2 * (int64_t)nq + y] != 0.0))
                     m |= M_MOVING;
-            } else if (crossed) {
+            } else if (crossed_ij || (crossed_k && d.bc[4] != PERIODIC)) {
                 m |= M_BND;
+            } else if (crossed_k) {
+                m |= M_KWRAP;
             }
         }
         for (int fc = 0; fc < 6; ++fc) {
@@ -689,6 +705,7 @@ __global__ void k_mask(const uint8_t* phase, const double* vel, uint32_t* mask, 
             if (fx && x == ((fc & 1) ? n - 1 : 0)) m |= M_ZH0 << fc;
         }
     }
+    if (m & M_BND) m &= ~M_KWRAP;  // resolved in full by the special-site kernel
     mask[c] = m;
     const unsigned long long notplain = __ballot(m != M_FLUID);
     if ((threadIdx.x & 63) == 0) wclass[c >> 6] = notplain ? 1 : 0;
@@ -965,6 +982,7 @@ struct lb_solver {
     double scale;
     hipStream_t stream = nullptr;
     uint64_t xcd_cells = LB_XCD_REMAP_CELLS;  // tile order threshold (tile_index)
+    uint32_t step_lds[2] = {0, 0};            // see StepParams::lds
     uint32_t* mov_idx = nullptr;  // moving-site index per site (k_ladd_prep)
     double* ladd_u = nullptr;     // 18 Ladd velocity sums per moving site
     size_t ladd_cap = 0;          // moving sites ladd_u has room for
@@ -1051,6 +1069,8 @@ static StepParams params(const lb_solver* s) {
     p.nq = s->nq;
     p.ntiles = (s->nq + TILE - 1) / TILE;
     p.xcd = s->nq >= s->xcd_cells ? 1u : 0u;
+    p.lds[0] = s->step_lds[0];
+    p.lds[1] = s->step_lds[1];
     p.rs = s->rs;
     p.ps = s->ps;
     p.wclass = s->wclass;
@@ -1264,8 +1284,8 @@ static void launch_init(lb_solver* s) {
 template <int P, int RLX>
 static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
-    if (odd) k_step<P, RLX, true><<<grid, TILE, 0, st>>>(p);
-    else k_step<P, RLX, false><<<grid, TILE, 0, st>>>(p);
+    if (odd) k_step<P, RLX, true><<<grid, TILE, p.lds[1], st>>>(p);
+    else k_step<P, RLX, false><<<grid, TILE, p.lds[0], st>>>(p);
     if (p.nspecial) {
         const unsigned g2 = (p.nspecial + TILE - 1) / TILE;
         if (odd) k_step_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
@@ -1341,6 +1361,13 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     s->ncell = (int64_t)s->d.nx * s->d.ny * s->d.nz;
     s->prec = cfg->precision;
     if (const char* env = getenv("LB_XCD_REMAP_CELLS")) s->xcd_cells = strtoull(env, nullptr, 10);
+    // Unused dynamic LDS caps the bulk step's occupancy. The FP32 kernel needs
+    // few registers (8 waves per SIMD); its even step streamed 3-4% faster at
+    // 6 waves per SIMD (26 KB per workgroup of 256 threads), while capping the
+    // other variants or the odd step measured slower.
+    if (s->prec == LB_FP32_SHIFTED) s->step_lds[0] = 26000;
+    if (const char* env = getenv("LB_STEP_LDS_EVEN")) s->step_lds[0] = (uint32_t)strtoul(env, nullptr, 10);
+    if (const char* env = getenv("LB_STEP_LDS_ODD")) s->step_lds[1] = (uint32_t)strtoul(env, nullptr, 10);
     s->rho0 = cfg->rho0 > 0 ? cfg->rho0 : 1.0;
     s->scale = cfg->fp16_scale > 0 ? cfg->fp16_scale : 1.0;
     // relaxation parameters, isothermal.pyx:273-287

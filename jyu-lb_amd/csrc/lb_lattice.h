@@ -62,6 +62,10 @@ constexpr uint32_t M_MOVING = 1u << 21;
 constexpr uint32_t M_ZH0 = 1u << 22;  // faces 0..5 -> bits 22..27
 constexpr uint32_t M_ZH_ANY = 0x3Fu << 22;
 constexpr uint32_t M_LINKS = (1u << 19) - 1;
+// the site lies on a periodic z face (k = 0 or nz - 1) and crosses no other
+// periodic/symmetric face: its z-crossing links wrap within the lattice row,
+// which the bulk kernel does with a uniform row-start address
+constexpr uint32_t M_KWRAP = 1u << 28;
 // sites handled by the special-site kernel (see lb_engine.hip)
 constexpr uint32_t M_SPECIAL = M_BND | M_ZH_ANY;
 

@@ -121,3 +121,48 @@ def test_gpu_fp64_bit_exact_with_xcd_tile_order(name, monkeypatch):
     monkeypatch.setenv("LB_XCD_REMAP_CELLS", "0")
     path = os.path.join(os.path.dirname(__file__), "golden", name + ".npz")
     test_gpu_fp64_bit_exact(path, 0)
+
+
+# Reduced-precision variants on every reference scenario: all boundary types
+# (periodic, walls, symmetric, Zou-He on each face, moving walls, mutations
+# between evolve calls) through the shifted-population forms. The density
+# (about 1) is compared in absolute terms, the velocity relative to its
+# largest magnitude in the reference field. Most scenarios run 0-60 steps
+# from random states; the gravity-driven examples run 20 calls of 500 steps
+# to a near-steady state, where the error settles at about one storage ulp
+# times the slow-mode relaxation time. Measured worst cases over all
+# scenarios and calls:
+#   FP32 arithmetic: density 1.5e-7, velocity 4.1e-5 (relative);
+#   FP32 storage / FP64 arithmetic: 6.5e-9, 5.8e-6;
+#   FP16 storage (11-bit mantissa): 3.0e-5, 4.3e-2 (gravity examples).
+_RED_TOL = {"fp32": (5e-6, 1e-4), "fp32_storage": (1e-7, 2e-5), "fp16": (5e-3, 1e-1)}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp32_storage"])
+@pytest.mark.parametrize("path", [p for p in GOLD if "c1_poiseuille" not in p],
+                         ids=[os.path.basename(p)[:-4] for p in GOLD if "c1_poiseuille" not in p])
+def test_reduced_precision_on_reference_scenarios(path, precision):
+    z = np.load(path)
+    sc = json.loads(str(z["scenario"]))
+    Solver = _solver_cls()
+
+    def mk(*a):
+        return Solver(*a, precision=precision)
+
+    out = scenarios.run_scenario(mk, sc, record_f=False)
+    s = z["sites"].astype(np.int64) + 1
+    tol_den, tol_vel = _RED_TOL[precision]
+    worst_den, worst_vel = 0.0, 0.0
+    for k in z.files:
+        if not (k.startswith("den_") or k.startswith("vel_")):
+            continue
+        a = z[k][s[0], s[1], s[2]]
+        b = out[k][s[0], s[1], s[2]]
+        err = float(np.max(np.abs(a - b)))
+        if k.startswith("den_"):
+            worst_den = max(worst_den, err)
+        else:
+            worst_vel = max(worst_vel, err / max(float(np.max(np.abs(a))), 1e-300))
+    print(f"{sc['name']} {precision}: worst density abs err {worst_den:.2e}, velocity rel err {worst_vel:.2e}",
+          flush=True)
+    assert worst_den <= tol_den and worst_vel <= tol_vel, (sc["name"], precision, worst_den, worst_vel)

@@ -31,7 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "jyu-lb_amd"))
 METRIC = "MLUPS D3Q19 AA fp64 vs fp32-shifted; adjoint/forward step time ratio"
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_SITE = {"fp64": 2 * 19 * 8, "fp32": 2 * 19 * 4, "fp16": 2 * 19 * 2, "fp32_storage": 2 * 19 * 4}
-KERNEL = {"fp64": "k_step<0,1,*>", "fp32": "k_step<1,1,*>", "fp16": "k_step<2,1,*>", "fp32_storage": "k_step<3,1,*>"}
+KERNEL = {"fp64": "k_step<0,1,*>", "fp32": "k_step<1,1,*>", "fp16": "k_step_h<1,*>", "fp32_storage": "k_step<3,1,*>"}
 
 
 def cavity_inputs(n):

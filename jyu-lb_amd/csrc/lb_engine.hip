@@ -330,6 +330,9 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
         return MODE == MODE_BULK && (m & M_KWRAP) && ((CK[l] > 0 && kk == 0) || (CK[l] < 0 && kk == nzm));
     };
     auto upw = [&](int l) { return up(l) + CK[l] + (CK[l] > 0 ? nzm : 0u); };
+    // wave-uniform: does any lane wrap? Other waves run the pull and store
+    // loops without the per-lane wrap tests.
+    const bool anyw = MODE == MODE_BULK && ODD && __ballot((m & M_KWRAP) != 0) != 0;
     constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
     constexpr bool NTLO = (LB_NT_MASK & 4) != 0, NTSO = (LB_NT_MASK & 8) != 0;
 
@@ -339,18 +342,23 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l) + kk, p.inv_scale);
     } else if constexpr (PLAIN) {
+        auto pull = [&](auto wrap) {
+            constexpr bool WRAP = decltype(wrap)::value;
 #pragma unroll
-        for (int l = 0; l < Q; ++l) {
-            // k-shifted pulls straddle cache lines shared with the next wave:
-            // LB_NT_MASK bit 4 keeps those cached and streams the rest
-            constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTLO;
-            if (l == REST) f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
-            else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
-                if (CK[l] == 0) f[l] = ldf<P, NTK>(up(l) + kk, p.inv_scale);
-                else if (wraps(l)) f[l] = ldf<P, NTLO>(upw(l), p.inv_scale);
-                else f[l] = ldf<P, NTLO>(up(l) + kk, p.inv_scale);
-            } else f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
-        }
+            for (int l = 0; l < Q; ++l) {
+                // k-shifted pulls straddle cache lines shared with the next wave:
+                // LB_NT_MASK bit 4 keeps those cached and streams the rest
+                constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTLO;
+                if (l == REST) f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
+                else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
+                    if (CK[l] == 0) f[l] = ldf<P, NTK>(up(l) + kk, p.inv_scale);
+                    else if (WRAP && wraps(l)) f[l] = ldf<P, NTLO>(upw(l), p.inv_scale);
+                    else f[l] = ldf<P, NTLO>(up(l) + kk, p.inv_scale);
+                } else f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
+            }
+        };
+        if (anyw) pull(std::true_type{});
+        else pull(std::false_type{});
     } else if constexpr (!ODD) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(F + e + (int64_t)l * p.ps, p.inv_scale);
@@ -409,16 +417,21 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)) + kk, f[l], p.scale);
     } else if constexpr (PLAIN) {
+        auto push = [&](auto wrap) {
+            constexpr bool WRAP = decltype(wrap)::value;
 #pragma unroll
-        for (int l = 0; l < Q; ++l) {
-            constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTSO;
-            if (l == REST) stf<P, NTK>(own(l) + kk, f[l], p.scale);
-            else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
-                if (CK[l] == 0) stf<P, NTK>(up(l) + kk, f[opp(l)], p.scale);
-                else if (wraps(l)) stf<P, NTSO>(upw(l), f[opp(l)], p.scale);
-                else stf<P, NTSO>(up(l) + kk, f[opp(l)], p.scale);
-            } else stf<P, NTK>(own(l) + kk, f[opp(l)], p.scale);
-        }
+            for (int l = 0; l < Q; ++l) {
+                constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTSO;
+                if (l == REST) stf<P, NTK>(own(l) + kk, f[l], p.scale);
+                else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
+                    if (CK[l] == 0) stf<P, NTK>(up(l) + kk, f[opp(l)], p.scale);
+                    else if (WRAP && wraps(l)) stf<P, NTSO>(upw(l), f[opp(l)], p.scale);
+                    else stf<P, NTSO>(up(l) + kk, f[opp(l)], p.scale);
+                } else stf<P, NTK>(own(l) + kk, f[opp(l)], p.scale);
+            }
+        };
+        if (anyw) push(std::true_type{});
+        else push(std::false_type{});
     } else if constexpr (!ODD) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSE>(F + e + (int64_t)opp(l) * p.ps, f[l], p.scale);
@@ -467,8 +480,10 @@ __global__ __launch_bounds__(TILE) LB_OCC void k_step(StepParams p) {
     step_bulk<P, RLX, ODD>(p);
 }
 
-// FP16 storage: the same kernel held to 64 VGPRs (8 waves per SIMD), which
-// it reaches without spills (69 VGPRs otherwise, 7 waves; measured ~4% slower).
+// FP16 storage with TRT: the same kernel held to 64 VGPRs (8 waves per SIMD),
+// which it reaches without spills (69 VGPRs otherwise, 7 waves; ~4% slower on
+// the TRT cavity). BGK and the regularized scheme measured 9-10% slower with
+// the hint (128x64x64 porous channel, 512x128x128 duct) and keep k_step.
 template <int RLX, bool ODD>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_h(StepParams p) {
     step_bulk<LB_FP16_STORAGE, RLX, ODD>(p);
@@ -1296,7 +1311,7 @@ static void launch_init(lb_solver* s) {
 template <int P, int RLX>
 static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
-    if constexpr (P == LB_FP16_STORAGE) {
+    if constexpr (P == LB_FP16_STORAGE && RLX == TRT) {
         if (odd) k_step_h<RLX, true><<<grid, TILE, p.lds[1], st>>>(p);
         else k_step_h<RLX, false><<<grid, TILE, p.lds[0], st>>>(p);
     } else {

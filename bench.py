@@ -140,7 +140,7 @@ def porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02):
     return s, nfl
 
 
-def porous_medium(precision, lat=(768, 384, 384), seed=5):
+def porous_medium(precision, lat=(768, 384, 384), seed=5, layout="rows"):
     """Throughput workload: seeded random-sphere porous medium (radius 6..12,
     20 % solid), TRT tau = 0.6, gravity gx = 1e-6, periodic on all faces
     (113M lattice cells)."""
@@ -148,7 +148,7 @@ def porous_medium(precision, lat=(768, 384, 384), seed=5):
     from jyulb.geometry import porous_spheres
     from jyulb.defs import TRT, PERIODIC
     s = Solver(np.array(lat, dtype=np.int32), TRT, (0.6 - 0.5) / 3.0, np.array([1e-6, 0.0, 0.0]),
-               np.full(6, PERIODIC, dtype=np.int32), precision=precision)
+               np.full(6, PERIODIC, dtype=np.int32), precision=precision, layout=layout)
     solid = porous_spheres(lat, 6, 12, 0.20, seed)
     s.get_phase()[...] = np.where(solid, 0, 1)
     del solid
@@ -156,16 +156,19 @@ def porous_medium(precision, lat=(768, 384, 384), seed=5):
     return s, nfl
 
 
-def throughput_workload(steps, precs):
-    """The 768x384x384 porous medium in each precision: MLUPS over the fluid
-    sites and over all lattice cells (HIP events around one evolve call)."""
+def throughput_workload(steps, precs, layouts=("rows",)):
+    """The 768x384x384 porous medium in each precision and layout: MLUPS over
+    the fluid sites and over all lattice cells (HIP events around one evolve
+    call). Entries are keyed by precision, with ":<layout>" for layouts other
+    than rows."""
     out = {}
     ncell = 768 * 384 * 384
-    for prec in precs:
-        s, nfl = porous_medium(prec)
-        print(f"[bench] porous 768x384x384 {prec}: {nfl} fluid sites, {steps} steps", file=sys.stderr, flush=True)
+    for layout, prec in [(lay, pr) for lay in layouts for pr in precs]:
+        s, nfl = porous_medium(prec, layout=layout)
+        key = prec if layout == "rows" else f"{prec}:{layout}"
+        print(f"[bench] porous 768x384x384 {key}: {nfl} fluid sites, {steps} steps", file=sys.stderr, flush=True)
         t, _ = time_device(s, steps, 10)
-        out[prec] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
+        out[key] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
                          ms_per_step=round(1e3 * t / steps, 4),
                          achieved_gbs=round(BYTES_PER_SITE[prec] * nfl * steps / t / 1e9, 1),
                          device_bytes=int(s.engine.info()["device_bytes"]))
@@ -299,6 +302,8 @@ def main():
     ap.add_argument("--duct-steps", type=int, default=5000, help="steps of the precision workload (0: skip)")
     ap.add_argument("--porous768-steps", type=int, default=0,
                     help="steps of the 768x384x384 porous-medium throughput workload (0: skip, the default)")
+    ap.add_argument("--porous768-layouts", default="rows",
+                    help="population layouts of the porous-medium workload, e.g. rows,sparse")
     ap.add_argument("--adj-interval", type=int, default=50)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain device copy of 2 GiB (torch copy_), the reference "
@@ -354,7 +359,8 @@ def main():
                              vel_linf=float(np.max(np.abs(d))),
                              vel_rel_linf=float(np.max(np.abs(d)) / np.max(np.abs(r))))
     duct_res = duct_precision(args.duct_steps, precs) if args.duct_steps > 0 else None
-    big = throughput_workload(args.porous768_steps, precs) if args.porous768_steps > 0 else None
+    big = (throughput_workload(args.porous768_steps, precs, args.porous768_layouts.split(","))
+           if args.porous768_steps > 0 else None)
     adj = None
     if args.adj_steps > 0:
         print(f"[bench] adjoint: {args.adj_steps} steps, checkpoint every {args.adj_interval}",

@@ -201,14 +201,24 @@ struct StepParams {
     const uint32_t* mov_idx;  // per site: index of a moving site in ladd_u
     const double* ladd_u;     // per moving site: the 18 wall-velocity sums of Ladd's term
     uint32_t nwaves;
-    const uint32_t* special;  // sites with M_SPECIAL, ascending within each wave
-    uint32_t nspecial;
+    // sites with M_SPECIAL: those on a Zou-He face first (nzh of them), then
+    // the rest; ascending within each wave
+    const uint32_t* special;
+    uint32_t nspecial, nzh;
     void* rec;  // if set: (rho, u*) of this step, 4 * nq of the compute type (adjoint recompute)
+    // sparse layout (LB_LAYOUT_SPARSE), null otherwise: compact index of every
+    // site (fluid sites only), site of every compact index, and the odd-step
+    // location of link l of compact site n at tab[link_slot(l) * ps + n]
+    const uint32_t* cidx;
+    const uint32_t* site;
+    const uint32_t* tab;
+    uint32_t nf;  // fluid sites = compact sites the step processes
 };
 
 // expanded index of padded site c in the population array: population l of
 // site (i,j,k) lives at (i*ny + j)*rs + k + l*ps
 LBD int64_t eidx(const StepParams& p, uint32_t c) {
+    if (p.cidx) return p.cidx[c];  // sparse: population l of site c at cidx[c] + l*ps
     if (p.rs == p.d.nzp) return c;
     const uint32_t row = c / (uint32_t)p.d.nzp;
     return (int64_t)row * p.rs + (c - row * (uint32_t)p.d.nzp);
@@ -258,7 +268,7 @@ LBD double ladd_sum(const Dims& d, const double* vel, int64_t nq, int i, int j, 
 }
 
 // table slot of link l (the rest population has none)
-LBD constexpr int ladd_slot(int l) { return l < REST ? l : l - 1; }
+LBD constexpr int link_slot(int l) { return l < REST ? l : l - 1; }
 // sign of the first non-zero component of c_l
 LBD constexpr bool ladd_plus(int l) { return (CI[l] != 0 ? CI[l] : CJ[l] != 0 ? CJ[l] : CK[l]) > 0; }
 
@@ -271,7 +281,7 @@ LBD void ladd_apply(const StepParams& p, uint32_t c, uint32_t m, T* f) {
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         if (l == REST || !(m & (1u << l))) continue;
-        const T term = (nnz(l) == 1 ? cff1 : cff2) * (T)U[ladd_slot(l)];
+        const T term = (nnz(l) == 1 ? cff1 : cff2) * (T)U[link_slot(l)];
         f[l] = ladd_plus(l) ? f[l] + term : f[l] - term;
     }
 }
@@ -298,6 +308,23 @@ LBD void ladd_apply(const StepParams& p, uint32_t c, uint32_t m, T* f) {
 //                 kernel over a compact list so that these rarely taken paths
 //                 do not raise the register budget of the bulk kernel
 constexpr int MODE_PLAIN = 0, MODE_BULK = 1, MODE_SPECIAL = 2;
+
+// Zou-He faces of site (i,j,k), in the order 1X, X1, 1Y, Y1, 1Z, Z1
+// (isothermal.pyx:1280-1291), on the streamed populations f
+template <class TC, bool SH>
+LBD void zouhe_site(const StepParams& p, uint32_t m, int i, int j, int k, TC rho0, TC* f) {
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) {
+        if (!(m & (M_ZH0 << fc))) continue;
+        int64_t fi;
+        if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
+        else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
+        else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
+        TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
+        if (p.d.bc[fc] == FIXED_DEN) zh_fix_den<TC, SH>(fc, (TC)p.fden[fc][fi], vb, rho0, f);
+        else zh_fix_vel<TC, SH>(fc, vb, rho0, f);
+    }
+}
 
 template <int P, int RLX, bool ODD, int MODE>
 LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
@@ -372,20 +399,8 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     // ---- Ladd moving walls on bounce-back links (previous-step density) ----
     if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) ladd_apply<TC>(p, c, m, f);
 
-    // ---- Zou-He faces, order 1X, X1, 1Y, Y1, 1Z, Z1 (isothermal.pyx:1280-1291) ----
-    if (!PLAIN && (m & M_ZH_ANY)) {
-#pragma unroll
-        for (int fc = 0; fc < 6; ++fc) {
-            if (!(m & (M_ZH0 << fc))) continue;
-            int64_t fi;
-            if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
-            else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
-            else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
-            TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
-            if (p.d.bc[fc] == FIXED_DEN) zh_fix_den<TC, SH>(fc, (TC)p.fden[fc][fi], vb, rho0, f);
-            else zh_fix_vel<TC, SH>(fc, vb, rho0, f);
-        }
-    }
+    // ---- Zou-He faces ----
+    if (!PLAIN && (m & M_ZH_ANY)) zouhe_site<TC, SH>(p, m, i, j, k, rho0, f);
 
     // ---- acceleration and collision ----
     TC acc[3];
@@ -457,7 +472,14 @@ LBD uint32_t tile_index(const StepParams& p) {
     return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
 }
 
-// Bulk step: every fluid site that is not special.
+// Mask bits that send a site to the special-site kernel. An even step reads
+// and writes only the site's own slots, so sites whose links cross a periodic
+// or symmetric face (M_BND) need no address resolution then and stay in the
+// bulk kernel; Zou-He sites need the face data on both parities.
+template <bool ODD>
+LBD constexpr uint32_t special_bits() { return ODD ? M_SPECIAL : M_ZH_ANY; }
+
+// Bulk step: every fluid site that is not special for this parity.
 template <int P, int RLX, bool ODD>
 LBD void step_bulk(const StepParams& p) {
     const uint32_t tile = tile_index(p);
@@ -471,7 +493,7 @@ LBD void step_bulk(const StepParams& p) {
         return;
     }
     const uint32_t m = p.mask[c];
-    if ((m & (M_FLUID | M_SPECIAL)) != M_FLUID) return;
+    if ((m & (M_FLUID | special_bits<ODD>())) != M_FLUID) return;
     step_site<P, RLX, ODD, MODE_BULK>(p, c, m);
 }
 
@@ -493,9 +515,105 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
 template <int P, int RLX, bool ODD>
 __global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
     const uint32_t t = blockIdx.x * TILE + threadIdx.x;
-    if (t >= p.nspecial) return;
+    if (t >= (ODD ? p.nspecial : p.nzh)) return;
     const uint32_t c = p.special[t];
     step_site<P, RLX, ODD, MODE_SPECIAL>(p, c, p.mask[c]);
+}
+
+// ---------------------------------------------------------------------------
+// sparse layout: fluid sites only
+// ---------------------------------------------------------------------------
+// Compact site n (fluid sites in ascending dense order, so consecutive threads
+// mostly hold consecutive k of one lattice row) keeps population l at
+// f[l*ps + n]. An even step reads and writes only these slots. An odd step
+// reads link l from tab[link_slot(l)*ps + n], the element that the dense
+// resolver (odd_loc) names for it, mapped to compact indices at
+// lb_finalize: upstream slot, own slot for bounce-back, or the periodic /
+// mirrored source; it writes f*_opp(l) back there. FULL sites may carry
+// moving-wall or Zou-He bits, read from the dense mask; lattices with
+// neither run the FULL = false kernel, which loads no mask.
+#ifndef LB_SP_RELOAD
+#define LB_SP_RELOAD 1
+#endif
+template <int P, int RLX, bool ODD, bool FULL>
+LBD void step_sparse(const StepParams& p, const uint32_t n) {
+    using TS = typename Var<P>::TS;
+    using TC = typename Var<P>::TC;
+    constexpr bool SH = Var<P>::SH;
+    TS* F = (TS*)p.f;
+    const uint32_t ps = (uint32_t)p.ps;  // 19 * ps < 2^32 (lb_finalize)
+    const TC* A = (const TC*)p.acc;
+    const bool need_c = FULL || A || p.store_dv || p.rec;  // uniform
+    const uint32_t c = need_c ? p.site[n] : 0u;
+    const uint32_t m = FULL ? p.mask[c] : M_FLUID;
+    constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
+    constexpr bool NTLO = (LB_NT_MASK & 4) != 0, NTSO = (LB_NT_MASK & 8) != 0;
+
+    // population l of this site: a wave-uniform base plus the 32-bit lane
+    // offset n (n * sizeof(TS) < 2^32)
+    auto own = [&](int l) { return uniform_ptr(F + (int64_t)l * ps) + n; };
+    TC f[Q];
+    uint32_t loc[Q];
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l), p.inv_scale);
+    } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) loc[l] = l == REST ? REST * ps + n : p.tab[link_slot(l) * ps + n];
+#pragma unroll
+        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLO>(l == REST ? own(l) : F + loc[l], p.inv_scale);
+    }
+    const TC rho0 = (TC)p.rho0;
+    if (FULL && p.moving && (m & M_MOVING)) ladd_apply<TC>(p, c, m, f);
+    if (FULL && (m & M_ZH_ANY)) {
+        int i, j, k;
+        decompose(c, p.d.ny, p.d.nzp, i, j, k);
+        zouhe_site<TC, SH>(p, m, i, j, k, rho0, f);
+    }
+    const int64_t nq = p.nq;
+    TC acc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
+    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
+    TC vel[3];
+    const TC den = collide<RLX, TC, SH>(f, acc, prm, rho0, vel);
+    if (p.store_dv) {
+        p.den[c] = (double)den;
+        p.vel[c] = (double)vel[0];
+        p.vel[nq + c] = (double)vel[1];
+        p.vel[2 * nq + c] = (double)vel[2];
+    } else if (FULL && p.moving && (m & M_MOVING)) {
+        p.den[c] = (double)den;
+    }
+    if (p.rec) {
+        TC* R = (TC*)p.rec;
+        R[c] = den;
+        R[nq + c] = vel[0];
+        R[2 * nq + c] = vel[1];
+        R[3 * nq + c] = vel[2];
+    }
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)), f[l], p.scale);
+    } else {
+#if LB_SP_RELOAD
+        // the table entries again (cache hits): holding 18 locations across
+        // the collision would cost the FP64 kernel a wave per SIMD
+#pragma unroll
+        for (int l = 0; l < Q; ++l) loc[l] = l == REST ? 0u : __builtin_nontemporal_load(p.tab + link_slot(l) * ps + n);
+#endif
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P, NTSO>(l == REST ? own(l) : F + loc[l], f[opp(l)], p.scale);
+    }
+}
+
+template <int P, int RLX, bool ODD, bool FULL>
+__global__ __launch_bounds__(TILE) void k_step_sp(StepParams p) {
+    const uint32_t tile = tile_index(p);
+    if (tile >= p.ntiles) return;
+    const uint32_t n = tile * TILE + threadIdx.x;
+    if (n >= p.nf) return;
+    step_sparse<P, RLX, ODD, FULL>(p, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -621,7 +739,7 @@ __global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
         return;
     }
     const uint32_t m = p.mask[c];
-    if ((m & (M_FLUID | M_SPECIAL)) != M_FLUID) return;
+    if ((m & (M_FLUID | special_bits<ODD>())) != M_FLUID) return;
     adj_site<P, RLX, ODD, MODE_BULK>(p, x, c, m);
 }
 
@@ -629,7 +747,7 @@ __global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
 template <int P, int RLX, bool ODD>
 __global__ __launch_bounds__(TILE) void k_adj_special(StepParams p, AdjParams x) {
     const uint32_t t = blockIdx.x * TILE + threadIdx.x;
-    if (t >= p.nspecial) return;
+    if (t >= (ODD ? p.nspecial : p.nzh)) return;
     const uint32_t c = p.special[t];
     adj_site<P, RLX, ODD, MODE_SPECIAL>(p, x, c, p.mask[c]);
 }
@@ -838,7 +956,11 @@ __global__ void k_post_f(StepParams p, int next_odd, double* out) {
         double v = 0.0;
         if (m & M_FLUID) {
             const int64_t e = eidx(p, c);
-            const int64_t loc = next_odd ? e + (int64_t)opp(l) * p.ps : odd_loc(p, m, e, opp(l), i, j, k);
+            const int lo = opp(l);
+            int64_t loc;
+            if (next_odd) loc = e + (int64_t)lo * p.ps;
+            else if (!p.tab) loc = odd_loc(p, m, e, lo, i, j, k);
+            else loc = lo == REST ? e + (int64_t)REST * p.ps : p.tab[(int64_t)link_slot(lo) * p.ps + e];
             v = (double)ldf<P, false>(F + loc, p.inv_scale);
             if (Var<P>::SH) v += (l == REST ? 12.0 / 36.0 : nnz(l) == 1 ? 2.0 / 36.0 : 1.0 / 36.0) * p.rho0;
         }
@@ -1018,6 +1140,14 @@ struct lb_solver {
     uint8_t* wclass = nullptr;
     uint32_t* special = nullptr;  // compact list of special sites
     uint32_t nspecial = 0, special_cap = 0;
+    uint32_t nzh = 0;  // Zou-He sites at the head of the special list
+    // sparse layout: compact index per site, site per compact index, odd-step
+    // table; nfp = fluid sites rounded up to 64 (the population stride)
+    bool sparse = false;
+    uint32_t* cidx = nullptr;
+    uint32_t* site = nullptr;
+    uint32_t* tab = nullptr;
+    uint32_t nfp = 0;
     uint8_t* phase = nullptr;
     double* den = nullptr;
     double* vel = nullptr;
@@ -1072,6 +1202,13 @@ static int dalloc(lb_solver* s, void** p, size_t n) {
     return LB_OK;
 }
 
+static void dfree(lb_solver* s, void** p, size_t n) {
+    if (!*p) return;
+    (void)hipFree(*p);
+    *p = nullptr;
+    s->bytes -= n;
+}
+
 static StepParams params(const lb_solver* s) {
     StepParams p;
     memset(&p, 0, sizeof(p));
@@ -1094,7 +1231,7 @@ static StepParams params(const lb_solver* s) {
     p.NJ = s->NJ;
     p.NK = s->NK;
     p.nq = s->nq;
-    p.ntiles = (s->nq + TILE - 1) / TILE;
+    p.ntiles = ((s->sparse ? s->nfp : s->nq) + TILE - 1) / TILE;
     p.xcd = s->nq >= s->xcd_cells ? 1u : 0u;
     p.lds[0] = s->step_lds[0];
     p.lds[1] = s->step_lds[1];
@@ -1104,10 +1241,17 @@ static StepParams params(const lb_solver* s) {
     p.nwaves = s->nq / 64;
     p.special = s->special;
     p.nspecial = s->nspecial;
+    p.nzh = s->nzh;
 
     p.moving = s->moving ? 1 : 0;
     p.mov_idx = s->mov_idx;
     p.ladd_u = s->ladd_u;
+    if (s->sparse) {
+        p.cidx = s->cidx;
+        p.site = s->site;
+        p.tab = s->tab;
+        p.nf = (uint32_t)s->nfluid;
+    }
     return p;
 }
 
@@ -1134,7 +1278,7 @@ static int check_config(const lb_config* c) {
     if ((fx[0] && (fx[1] || fx[2])) || (fx[1] && (fx[0] || fx[2])) || (fx[2] && (fx[0] || fx[1])))
         return fail(LB_EINVAL, "Invalid fixed variable bc setup.");
     if (c->precision < LB_FP64 || c->precision > LB_FP32_STORAGE) return fail(LB_EINVAL, "Invalid precision.");
-    if (c->layout < LB_LAYOUT_ROWS || c->layout > LB_LAYOUT_SOA_PADDED) return fail(LB_EINVAL, "Invalid layout.");
+    if (c->layout < LB_LAYOUT_ROWS || c->layout > LB_LAYOUT_SPARSE) return fail(LB_EINVAL, "Invalid layout.");
     const int64_t nzp = (c->lat[2] + 63) / 64 * 64;
     const int64_t nq = (int64_t)c->lat[0] * c->lat[1] * nzp;
     if (nq >= (int64_t)1 << 31) return fail(LB_EINVAL, "Lattice too large (%lld padded sites).", (long long)nq);
@@ -1166,32 +1310,35 @@ static int build_mask(lb_solver* s, bool with_walls) {
     return LB_OK;
 }
 
-// count fluid, moving-wall and special sites from the mask
+// count fluid, moving-wall, special and Zou-He sites from the mask
 __global__ void k_count(const uint32_t* mask, uint32_t nq, unsigned long long* out) {
-    __shared__ unsigned long long sh[3];
-    if (threadIdx.x < 3) sh[threadIdx.x] = 0;
+    __shared__ unsigned long long sh[4];
+    if (threadIdx.x < 4) sh[threadIdx.x] = 0;
     __syncthreads();
-    unsigned long long a = 0, b = 0, c3 = 0;
+    unsigned long long a = 0, b = 0, c3 = 0, c4 = 0;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nq; c += gridDim.x * blockDim.x) {
         const uint32_t m = mask[c];
         a += (m & M_FLUID) ? 1 : 0;
         b += (m & M_MOVING) ? 1 : 0;
         c3 += ((m & M_FLUID) && (m & M_SPECIAL)) ? 1 : 0;
+        c4 += ((m & M_FLUID) && (m & M_ZH_ANY)) ? 1 : 0;
     }
     atomicAdd(&sh[0], a);
     atomicAdd(&sh[1], b);
     atomicAdd(&sh[2], c3);
+    atomicAdd(&sh[3], c4);
     __syncthreads();
-    if (threadIdx.x < 3) atomicAdd(&out[threadIdx.x], sh[threadIdx.x]);
+    if (threadIdx.x < 4) atomicAdd(&out[threadIdx.x], sh[threadIdx.x]);
 }
 
-// compact list of the special fluid sites: one atomic per wave, ascending
-// order within a wave
-__global__ void k_list_special(const uint32_t* mask, uint32_t nq, uint32_t* list, unsigned int* counter) {
+// compact list of the fluid sites with a bit of `need` and none of `skip`,
+// appended at *counter: one atomic per wave, ascending order within a wave
+__global__ void k_list_special(const uint32_t* mask, uint32_t nq, uint32_t need, uint32_t skip, uint32_t* list,
+                               unsigned int* counter) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nq) return;  // whole waves only: nq % 64 == 0
     const uint32_t m = mask[c];
-    const bool sp = (m & M_FLUID) && (m & M_SPECIAL);
+    const bool sp = (m & M_FLUID) && (m & need) && !(m & skip);
     const unsigned long long b = __ballot(sp);
     if (!b) return;
     const int lane = threadIdx.x & 63;
@@ -1231,16 +1378,67 @@ __global__ void k_ladd_prep(const uint32_t* mask, const double* vel, Dims d, uin
     ladd_sums(d, vel, nq, i, j, k, m, tmp, std::make_integer_sequence<int, Q>{});
 #pragma unroll
     for (int l = 0; l < Q; ++l)
-        if (l != REST) out[ladd_slot(l)] = tmp[l];
+        if (l != REST) out[link_slot(l)] = tmp[l];
+}
+
+// sparse layout set-up: fluid sites per wave of 64 sites
+__global__ void k_wave_fluid(const uint32_t* mask, uint32_t nq, uint32_t* cnt) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;  // whole waves only: nq % 64 == 0
+    const unsigned long long b = __ballot((mask[c] & M_FLUID) != 0);
+    if ((threadIdx.x & 63) == 0) cnt[c >> 6] = (uint32_t)__popcll(b);
+}
+
+// compact index of every fluid site from the per-wave offsets (ascending
+// dense order), the site of every compact index; solid sites get ~0u
+__global__ void k_compact(const uint32_t* mask, uint32_t nq, const uint32_t* woff, uint32_t* cidx, uint32_t* site) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq) return;
+    const bool fl = (mask[c] & M_FLUID) != 0;
+    const unsigned long long b = __ballot(fl);
+    const int lane = threadIdx.x & 63;
+    if (!fl) {
+        cidx[c] = ~0u;
+        return;
+    }
+    const uint32_t n = woff[c >> 6] + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+    cidx[c] = n;
+    site[n] = c;
+}
+
+// odd-step locations of every fluid site in compact indices. `q` addresses a
+// dense SoA view (element l*nq + c), in which odd_loc resolves bounce-back,
+// periodic, symmetric and z-wrapped links; a location on a non-fluid site
+// (which the link mask rules out) raises *err instead of being stored.
+__global__ void k_sparse_tab(StepParams q, const uint32_t* cidx, uint32_t nfp, uint32_t* tab, unsigned int* err) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= q.nq) return;
+    const uint32_t m = q.mask[c];
+    if (!(m & M_FLUID)) return;
+    int i, j, k;
+    decompose(c, q.d.ny, q.d.nzp, i, j, k);
+    const uint32_t n = cidx[c];
+    for (int l = 0; l < Q; ++l) {
+        if (l == REST) continue;
+        const int64_t e = odd_loc<true>(q, m, (int64_t)c, l, i, j, k);
+        const int64_t ls = e / q.nq, cs = e - ls * q.nq;
+        const uint32_t ns = (ls >= 0 && ls < Q && cs >= 0 && cs < q.nq) ? cidx[cs] : ~0u;
+        if (ns == ~0u) {
+            atomicOr(err, 1u);
+            tab[(int64_t)link_slot(l) * nfp + n] = (uint32_t)REST * nfp + n;  // a valid location
+        } else {
+            tab[(int64_t)link_slot(l) * nfp + n] = (uint32_t)ls * nfp + ns;
+        }
+    }
 }
 
 static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
     unsigned long long* d = (unsigned long long*)s->red;
-    HIPCHK(hipMemsetAsync(d, 0, 24, s->stream));
+    HIPCHK(hipMemsetAsync(d, 0, 32, s->stream));
     k_count<<<256, 256, 0, s->stream>>>(s->mask, s->nq, d);
     HIPCHK(hipGetLastError());
-    unsigned long long h[3];
-    HIPCHK(hipMemcpyAsync(h, d, 24, hipMemcpyDeviceToHost, s->stream));
+    unsigned long long h[4];
+    HIPCHK(hipMemcpyAsync(h, d, 32, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     *nfluid = (int64_t)h[0];
     *nmoving = (int64_t)h[1];
@@ -1256,10 +1454,15 @@ static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
         s->special_cap = nsp;
     }
     s->nspecial = nsp;
+    s->nzh = (uint32_t)h[3];
     if (nsp > 0) {
+        // Zou-He sites first (both parities), then the rest (odd steps only)
         unsigned int* cnt = (unsigned int*)s->red;
         HIPCHK(hipMemsetAsync(cnt, 0, 4, s->stream));
-        k_list_special<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->nq, s->special, cnt);
+        k_list_special<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->nq, M_ZH_ANY, 0u, s->special,
+                                                                     cnt);
+        k_list_special<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->nq, M_SPECIAL, M_ZH_ANY,
+                                                                     s->special, cnt);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s->stream));
     }
@@ -1311,6 +1514,16 @@ static void launch_init(lb_solver* s) {
 template <int P, int RLX>
 static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    if (p.tab) {  // sparse layout: one kernel, with the mask only where needed
+        if (p.moving || p.nzh) {
+            if (odd) k_step_sp<P, RLX, true, true><<<grid, TILE, 0, st>>>(p);
+            else k_step_sp<P, RLX, false, true><<<grid, TILE, 0, st>>>(p);
+        } else {
+            if (odd) k_step_sp<P, RLX, true, false><<<grid, TILE, 0, st>>>(p);
+            else k_step_sp<P, RLX, false, false><<<grid, TILE, 0, st>>>(p);
+        }
+        return;
+    }
     if constexpr (P == LB_FP16_STORAGE && RLX == TRT) {
         if (odd) k_step_h<RLX, true><<<grid, TILE, p.lds[1], st>>>(p);
         else k_step_h<RLX, false><<<grid, TILE, p.lds[0], st>>>(p);
@@ -1318,8 +1531,9 @@ static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
         if (odd) k_step<P, RLX, true><<<grid, TILE, p.lds[1], st>>>(p);
         else k_step<P, RLX, false><<<grid, TILE, p.lds[0], st>>>(p);
     }
-    if (p.nspecial) {
-        const unsigned g2 = (p.nspecial + TILE - 1) / TILE;
+    const uint32_t ns = odd ? p.nspecial : p.nzh;
+    if (ns) {
+        const unsigned g2 = (ns + TILE - 1) / TILE;
         if (odd) k_step_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
         else k_step_special<P, RLX, false><<<g2, TILE, 0, st>>>(p);
     }
@@ -1346,8 +1560,9 @@ static void launch_adj(const lb_solver* s, const StepParams& p, const AdjParams&
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
     if (odd) k_adj<P, RLX, true><<<grid, TILE, 0, s->stream>>>(p, x);
     else k_adj<P, RLX, false><<<grid, TILE, 0, s->stream>>>(p, x);
-    if (p.nspecial) {
-        const unsigned g2 = (p.nspecial + TILE - 1) / TILE;
+    const uint32_t ns = odd ? p.nspecial : p.nzh;
+    if (ns) {
+        const unsigned g2 = (ns + TILE - 1) / TILE;
         if (odd) k_adj_special<P, RLX, true><<<g2, TILE, 0, s->stream>>>(p, x);
         else k_adj_special<P, RLX, false><<<g2, TILE, 0, s->stream>>>(p, x);
     }
@@ -1400,6 +1615,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     if (s->prec == LB_FP32_SHIFTED) s->step_lds[0] = 26000;
     if (const char* env = getenv("LB_STEP_LDS_EVEN")) s->step_lds[0] = (uint32_t)strtoul(env, nullptr, 10);
     if (const char* env = getenv("LB_STEP_LDS_ODD")) s->step_lds[1] = (uint32_t)strtoul(env, nullptr, 10);
+    for (uint32_t& b : s->step_lds) b = b > 65536u ? 65536u : b;  // 64 KiB keeps one workgroup per CU
     s->rho0 = cfg->rho0 > 0 ? cfg->rho0 : 1.0;
     s->scale = cfg->fp16_scale > 0 ? cfg->fp16_scale : 1.0;
     // relaxation parameters, isothermal.pyx:273-287
@@ -1428,7 +1644,10 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     }
     s->fsize = cfg->layout == LB_LAYOUT_SOA || cfg->layout == LB_LAYOUT_SOA_PADDED ? (int64_t)Q * s->ps
                                                                                   : rows * s->rs;
-    if ((rc = dalloc(s, &s->f, (size_t)s->fsize * ts_size(s->prec))) ||
+    // the sparse layout sizes its arrays from the fluid sites at lb_finalize
+    s->sparse = cfg->layout == LB_LAYOUT_SPARSE;
+    if (s->sparse) s->fsize = 0;
+    if ((!s->sparse && (rc = dalloc(s, &s->f, (size_t)s->fsize * ts_size(s->prec)))) ||
         (rc = dalloc(s, (void**)&s->mask, nq * 4)) || (rc = dalloc(s, (void**)&s->phase, nq)) ||
         (rc = dalloc(s, (void**)&s->wclass, nq / 64)) ||
         (rc = dalloc(s, (void**)&s->den, nq * 8)) || (rc = dalloc(s, (void**)&s->vel, 3 * nq * 8)) ||
@@ -1440,7 +1659,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
         return rc;
     }
     // den = 1, vel = 0, phase SOLID, mask 0 (isothermal.pyx:243-248)
-    (void)hipMemset(s->f, 0, (size_t)s->fsize * ts_size(s->prec));
+    if (s->f) (void)hipMemset(s->f, 0, (size_t)s->fsize * ts_size(s->prec));
     (void)hipMemset(s->mask, 0, nq * 4);
     (void)hipMemset(s->wclass, 1, nq / 64);
     (void)hipMemset(s->phase, 0, nq);
@@ -1479,7 +1698,7 @@ int lb_destroy(lb_solver* s) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
                     s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
-                    s->snap,  s->snap_dv, s->mov_idx, s->ladd_u};
+                    s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {
@@ -1563,15 +1782,86 @@ int lb_set_face_data(lb_solver* s, int face, const double* den_face, const doubl
     return LB_OK;
 }
 
+// Sparse layout: compact the fluid sites in ascending dense order, size the
+// population array to them (stride nfp) and tabulate every odd-step location.
+static int build_sparse(lb_solver* s) {
+    const uint64_t nf = (uint64_t)s->nfluid;
+    const uint64_t nfp = std::max<uint64_t>(64, (nf + 63) / 64 * 64);
+    if ((uint64_t)Q * nfp >= (1ull << 32))
+        return fail(LB_EINVAL, "sparse layout: %llu fluid sites exceed its 32-bit location table",
+                    (unsigned long long)nf);
+    const size_t ts = ts_size(s->prec);
+    int rc;
+    if (s->nfp != nfp) {
+        const size_t o = s->nfp;
+        dfree(s, &s->f, Q * o * ts);
+        dfree(s, &s->snap, Q * o * ts);
+        dfree(s, (void**)&s->site, o * 4);
+        dfree(s, (void**)&s->tab, (Q - 1) * o * 4);
+        s->nfp = 0;
+        if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
+            (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)))
+            return rc;
+        s->nfp = (uint32_t)nfp;
+    }
+    if (!s->cidx && (rc = dalloc(s, (void**)&s->cidx, (size_t)s->nq * 4))) return rc;
+    s->ps = (int64_t)nfp;
+    s->rs = 0;
+    s->fsize = (int64_t)Q * nfp;
+    HIPCHK(hipMemsetAsync(s->f, 0, Q * nfp * ts, s->stream));  // padding sites stay zero
+    HIPCHK(hipMemsetAsync(s->site, 0, nfp * 4, s->stream));
+    // fluid sites per wave -> exclusive scan on the host -> compact indices
+    const uint32_t nw = s->nq / 64;
+    uint32_t* woff = nullptr;
+    if ((rc = dalloc(s, (void**)&woff, (size_t)nw * 4))) return rc;
+    k_wave_fluid<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->nq, woff);
+    std::vector<uint32_t> h(nw);
+    hipError_t e = hipMemcpyAsync(h.data(), woff, (size_t)nw * 4, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    uint64_t run = 0;
+    for (uint32_t& v : h) {
+        const uint32_t t = v;
+        v = (uint32_t)run;
+        run += t;
+    }
+    if (e == hipSuccess && run == nf)
+        e = hipMemcpyAsync(woff, h.data(), (size_t)nw * 4, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess && run == nf) {
+        k_compact<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->nq, woff, s->cidx, s->site);
+        e = hipStreamSynchronize(s->stream);
+    }
+    dfree(s, (void**)&woff, (size_t)nw * 4);
+    if (e != hipSuccess) return fail(LB_EHIP, "sparse layout set-up: %s", hipGetErrorString(e));
+    if (run != nf) return fail(LB_EHIP, "sparse layout set-up: %llu fluid sites counted, %llu listed",
+                               (unsigned long long)nf, (unsigned long long)run);
+    // odd-step locations, resolved in a dense SoA view: element l*nq + c
+    StepParams q = params(s);
+    q.cidx = nullptr;
+    q.tab = nullptr;
+    q.rs = s->d.nzp;
+    q.ps = s->nq;
+    unsigned int* err = (unsigned int*)s->red;
+    HIPCHK(hipMemsetAsync(err, 0, 4, s->stream));
+    k_sparse_tab<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(q, s->cidx, (uint32_t)nfp, s->tab, err);
+    HIPCHK(hipGetLastError());
+    unsigned int herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (herr) return fail(LB_EHIP, "sparse layout set-up: a link resolves to a non-fluid site");
+    return LB_OK;
+}
+
 int lb_finalize(lb_solver* s, int64_t* nfluid) {
     if (!s) return fail(LB_EINVAL, "null handle");
     if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
+    s->finalized = false;  // until the populations below are initialised
     int rc = build_mask(s, true);
     if (rc) return rc;
     int64_t nf, nm;
     if ((rc = count_mask(s, &nf, &nm))) return rc;
     s->nfluid = nf;
     s->moving = nm > 0;
+    if (s->sparse && (rc = build_sparse(s))) return rc;
     if (s->prec == LB_FP16_STORAGE && !(s->cfg.fp16_scale > 0)) {
         k_maxdev<<<256, 256, 0, s->stream>>>(s->mask, s->den, s->vel, s->nq, s->rho0, s->part);
         HIPCHK(hipGetLastError());
@@ -1747,6 +2037,7 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
     if (s->prec != LB_FP64 && s->prec != LB_FP32_SHIFTED)
         return fail(LB_EINVAL, "the adjoint runs in FP64 or FP32-shifted precision");
     if (s->moving) return fail(LB_EINVAL, "the adjoint supports stationary walls only (no moving solid sites)");
+    if (s->sparse) return fail(LB_EINVAL, "the adjoint runs on the dense layouts (not LB_LAYOUT_SPARSE)");
     const uint32_t nseg = (steps + interval - 1) / interval;
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
     int rc;

@@ -66,7 +66,8 @@ constexpr uint32_t M_LINKS = (1u << 19) - 1;
 // periodic/symmetric face: its z-crossing links wrap within the lattice row,
 // which the bulk kernel does with a uniform row-start address
 constexpr uint32_t M_KWRAP = 1u << 28;
-// sites handled by the special-site kernel (see lb_engine.hip)
+// sites handled by the special-site kernel on odd steps; even steps touch only
+// the site's own slots, and send only the Zou-He sites there (see lb_engine.hip)
 constexpr uint32_t M_SPECIAL = M_BND | M_ZH_ANY;
 
 }  // namespace lb

@@ -19,6 +19,12 @@ PRECISIONS = {"fp64": LB_FP64, "fp32": LB_FP32_SHIFTED, "fp32_shifted": LB_FP32_
               LB_FP32_STORAGE: LB_FP32_STORAGE}
 
 
+LB_LAYOUT_ROWS, LB_LAYOUT_SOA, LB_LAYOUT_SOA_PADDED, LB_LAYOUT_SPARSE = 0, 1, 2, 3
+LAYOUTS = {"rows": LB_LAYOUT_ROWS, "soa": LB_LAYOUT_SOA, "soa_padded": LB_LAYOUT_SOA_PADDED,
+           "sparse": LB_LAYOUT_SPARSE}
+LAYOUTS.update({v: v for v in list(LAYOUTS.values())})
+
+
 class LbConfig(ctypes.Structure):
     _fields_ = [
         ("lat", ctypes.c_int32 * 3),
@@ -110,7 +116,12 @@ def ptr(a):
 class Engine:
     """Owner of one device solver handle (lb_solver*)."""
 
-    def __init__(self, lat, rlx, kvisc, g, bc, precision=LB_FP64, rho0=1.0, fp16_scale=0.0, layout=0):
+    def __init__(self, lat, rlx, kvisc, g, bc, precision=LB_FP64, rho0=1.0, fp16_scale=0.0, layout=LB_LAYOUT_ROWS):
+        if precision not in PRECISIONS:
+            raise ValueError(f"unknown precision {precision!r}; one of "
+                             f"{sorted(k for k in PRECISIONS if isinstance(k, str))}")
+        if layout not in LAYOUTS:
+            raise ValueError(f"unknown layout {layout!r}; one of {sorted(k for k in LAYOUTS if isinstance(k, str))}")
         L = lib()
         if L.lb_device_count() < 1:
             raise RuntimeError("jyulb: no HIP device available (the evolve path runs on the GPU only)")
@@ -125,7 +136,7 @@ class Engine:
         cfg.precision = int(PRECISIONS[precision])
         cfg.rho0 = float(rho0)
         cfg.fp16_scale = float(fp16_scale)
-        cfg.layout = int(layout)
+        cfg.layout = int(LAYOUTS[layout])
         h = ctypes.c_void_p()
         check(L.lb_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

@@ -18,6 +18,10 @@ Additions, keyword-only so that reference call sites are unaffected:
              storage of the scaled shifted populations, FP32 arithmetic) or
              "fp32_storage" (FP32 storage of the shifted populations, FP64
              arithmetic: the FP32 traffic with the rounding of storage only)
+  layout     population layout on the device: "rows" (default), "soa",
+             "soa_padded", or "sparse" (fluid sites only, with a table of the
+             odd-step locations: less traffic on porous media; no adjoint).
+             Results are identical in every layout.
 """
 import numpy as np
 
@@ -39,7 +43,7 @@ class Solver:
     # constructors
     # ------------------------------------------------------------------
     def __init__(self, lat_size, rlx_scheme=BGK, kvisc=1.0 / 6.0, g=None, bc=None, *,
-                 precision="fp64", rho0=1.0, fp16_scale=0.0, layout=0):
+                 precision="fp64", rho0=1.0, fp16_scale=0.0, layout="rows"):
         """isothermal.pyx:156-298"""
         self.error_msg = ''
         if lat_size is None:

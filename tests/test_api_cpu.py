@@ -72,6 +72,23 @@ def test_no_device_fails_loudly():
         Solver(np.array([4, 4, 4], dtype=np.int32), defs.TRT, 0.1)
 
 
+def test_variant_names_checked_before_the_device():
+    """Precision and layout names are validated on the host: every listed name
+    passes to the device step, an unknown one raises ValueError."""
+    import torch
+    from jyulb import _native
+    lat = np.array([4, 4, 4], dtype=np.int32)
+    for kw in ({"precision": "fp24"}, {"layout": "dense"}, {"layout": 7}):
+        with pytest.raises(ValueError, match="unknown"):
+            Solver(lat, defs.TRT, 0.1, **kw)
+    assert {k for k in _native.LAYOUTS if isinstance(k, str)} == {"rows", "soa", "soa_padded", "sparse"}
+    assert [_native.LAYOUTS[k] for k in ("rows", "soa", "soa_padded", "sparse")] == [0, 1, 2, 3]
+    if not torch.cuda.is_available():
+        for kw in ({"layout": "sparse"}, {"precision": "fp32_storage", "layout": 3}):
+            with pytest.raises(RuntimeError, match="no HIP device"):
+                Solver(lat, defs.TRT, 0.1, **kw)
+
+
 def test_flow_field_closed_forms():
     H, nu, pg = 10, 0.1, -2e-5
     dist, ana = np.zeros(H), np.zeros(H)

@@ -74,14 +74,16 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("layout", ["rows", "sparse"])
 @pytest.mark.parametrize("ci", range(len(CASES)))
-def test_gpu_matches_oracle_larger(ci):
+def test_gpu_matches_oracle_larger(ci, layout):
     import jyulb_oracle as jo
     bc, rlx, lat = CASES[ci]
     sc = _random_case(100 + ci, bc, rlx, lat, [3, 8, 21])
-    print(f"case {ci} lat={lat} bc={bc} rlx={rlx}", flush=True)
+    print(f"case {ci} lat={lat} bc={bc} rlx={rlx} layout={layout}", flush=True)
     ref = scenarios.run_scenario(jo.OracleSolver, sc)
-    got = scenarios.run_scenario(_solver_cls(), sc)
+    Solver = _solver_cls()
+    got = scenarios.run_scenario(lambda *a: Solver(*a, layout=layout), sc)
     for k in ref:
         if k in ("nfluid", "sites") or k.startswith("tsteps"):
             continue
@@ -91,7 +93,7 @@ def test_gpu_matches_oracle_larger(ci):
         _close(b, a, f"case {ci} {k}")
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2], ids=["rows", "soa", "soa_padded"])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3], ids=["rows", "soa", "soa_padded", "sparse"])
 @pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
 def test_gpu_fp64_bit_exact(path, layout):
     """The FP64 kernel keeps the reference's operation order and is built
@@ -114,13 +116,14 @@ def test_gpu_fp64_bit_exact(path, layout):
         np.testing.assert_array_equal(b, a, err_msg=f"{sc['name']} {k} layout {layout}")
 
 
+@pytest.mark.parametrize("layout", [0, 3], ids=["rows", "sparse"])
 @pytest.mark.parametrize("name", ["trt_periodic", "bgk_zouhe_x", "reg_symwall"])
-def test_gpu_fp64_bit_exact_with_xcd_tile_order(name, monkeypatch):
+def test_gpu_fp64_bit_exact_with_xcd_tile_order(name, layout, monkeypatch):
     """Large lattices deal the step kernel's tiles to the XCDs in contiguous
     ranges; forcing that order on small cases must not change a bit."""
     monkeypatch.setenv("LB_XCD_REMAP_CELLS", "0")
     path = os.path.join(os.path.dirname(__file__), "golden", name + ".npz")
-    test_gpu_fp64_bit_exact(path, 0)
+    test_gpu_fp64_bit_exact(path, layout)
 
 
 # Reduced-precision variants on every reference scenario: all boundary types

@@ -1,0 +1,160 @@
+"""Sparse layout (fluid sites only) against the dense layouts and the oracle.
+
+The sparse layout stores the populations of the fluid sites only, compacted in
+ascending dense order, and runs the odd AA step through a table of the 18
+locations per site that the dense resolver names (bounce-back, periodic,
+z-wrapped and mirrored links): the fluid-only arrays of the reference's C++
+kernel (include/kernel/kernel.cpp:227-271), here with 32-bit entries computed
+on the GPU at finalize_init. The per-site arithmetic is the dense kernel's, so
+every precision reproduces the dense ROWS layout bit for bit. FP64 against
+all reference fixtures runs in test_gpu_parity.py.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import scenarios
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+# every boundary kind: periodic, walls + symmetric, symmetric, Zou-He on x / y
+# / negative faces, moving walls (Couette), a long TRT run
+NAMES = ["bgk_periodic", "trt_symwall", "reg_symmetric", "reg_zouhe_x", "trt_zouhe_y", "bgk_zouhe_neg",
+         "ex_couette", "trt_long"]
+
+
+def _run_fixture(name, **kw):
+    from jyulb.isothermal import Solver
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    sc = json.loads(str(z["scenario"]))
+
+    def mk(*a):
+        return Solver(*a, **kw)
+
+    return scenarios.run_scenario(mk, sc, record_f=True)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "fp32_storage"])
+@pytest.mark.parametrize("name", NAMES)
+def test_sparse_equals_rows_bitwise(name, precision):
+    print(f"scenario {name} precision {precision}", flush=True)
+    a = _run_fixture(name, precision=precision)
+    b = _run_fixture(name, precision=precision, layout="sparse")
+    assert a.keys() == b.keys()
+    for k in a:
+        np.testing.assert_array_equal(b[k], a[k], err_msg=f"{name} {precision} {k}")
+
+
+def _porous_run(Solver, lat, steps, **kw):
+    from jyulb.defs import TRT, PERIODIC, SYMMETRIC, SOLID, FLUID
+    from jyulb.geometry import porous_spheres
+    lat = np.array(lat, dtype=np.int32)
+    bc = np.array([PERIODIC, PERIODIC, SYMMETRIC, SYMMETRIC, PERIODIC, PERIODIC], dtype=np.int32)
+    s = Solver(lat, TRT, 1.0 / 30.0, np.array([1e-5, 0.0, 2e-6]), bc, **kw)
+    solid = porous_spheres(lat, 3, 6, 0.25, seed=11)
+    ph = np.where(solid, SOLID, FLUID).astype(np.uint8)
+    s.get_phase()[...] = ph
+    rng = np.random.default_rng(4)
+    acc = np.asarray(s.get_acc())
+    acc[...] = 1e-6 * rng.standard_normal(acc.shape) * (ph != SOLID)[..., None]
+    nf = s.finalize_init()
+    out = [nf]
+    for st in steps:
+        s.evolve(st)
+        out += [np.array(s.get_den(), copy=True), np.array(s.get_vel(), copy=True)]
+    return out, ph
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp16"])
+def test_sparse_porous_medium_equals_rows(precision):
+    """Porous medium, 25% solid, periodic x/z and symmetric y faces, gravity
+    plus a per-site force, nz not a multiple of 64; odd and even step counts
+    over three evolve calls."""
+    from jyulb.isothermal import Solver
+    lat, steps = (72, 40, 90), (7, 40, 53)
+    print(f"porous {lat} {precision}", flush=True)
+    a, ph = _porous_run(Solver, lat, steps, precision=precision)
+    b, _ = _porous_run(Solver, lat, steps, precision=precision, layout="sparse")
+    assert a[0] == b[0] == int((ph != 0).sum())
+    for x, y in zip(a[1:], b[1:]):
+        np.testing.assert_array_equal(y, x)
+
+
+def test_sparse_porous_medium_matches_oracle():
+    import jyulb_oracle as jo
+    from jyulb.isothermal import Solver
+
+    def sp(*a):
+        return Solver(*a, layout="sparse")
+
+    lat, steps = (40, 24, 30), (5, 12)
+    ref, ph = _porous_run(jo.OracleSolver, lat, steps)
+    got, _ = _porous_run(sp, lat, steps)
+    fl = ph != 0
+    assert ref[0] == got[0]
+    for x, y in zip(ref[1:], got[1:]):
+        np.testing.assert_array_equal(y[fl], x[fl])  # interior views
+
+
+def test_sparse_fewer_fluid_sites_than_a_wave():
+    """27 fluid sites: the compact arrays are padded to one wave of 64."""
+    import jyulb_oracle as jo
+    from jyulb.isothermal import Solver
+    from jyulb.defs import TRT, PERIODIC
+    lat = np.array([3, 3, 3], dtype=np.int32)
+    bc = np.array([PERIODIC] * 6, dtype=np.int32)
+    res = []
+    for mk in (jo.OracleSolver, lambda *a: Solver(*a, layout="sparse")):
+        s = mk(lat, TRT, 0.1, np.array([1e-5, -2e-5, 3e-5]), bc)
+        s.get_phase()[...] = 1
+        np.asarray(s.get_vel())[..., 0] = 0.001 * np.arange(27).reshape(3, 3, 3)
+        assert s.finalize_init() == 27
+        s.evolve(9)
+        res.append((np.array(s.get_den()), np.array(s.get_vel())))
+    for x, y in zip(res[0], res[1]):
+        np.testing.assert_array_equal(y, x)
+
+
+def test_sparse_all_solid_lattice():
+    """No fluid site: evolve does nothing and, like the reference
+    (isothermal.pyx:1405-1406), returns the unchanged step count."""
+    from jyulb.isothermal import Solver
+    from jyulb.defs import BGK
+    s = Solver(np.array([8, 4, 5], dtype=np.int32), BGK, 0.1, layout="sparse")
+    assert s.finalize_init() == 0
+    assert s.evolve(4) == 0
+    assert np.all(np.asarray(s.get_den()) == 1.0)
+
+
+def test_sparse_device_memory_follows_fluid_sites():
+    """Population bytes scale with the fluid sites, not the lattice."""
+    from jyulb.isothermal import Solver
+    lat = (64, 32, 64)
+    a, ph = _porous_run(Solver, lat, (2,))
+    nf = a[0]
+    rows = Solver(np.array(lat, dtype=np.int32), 0, 0.1)
+    rows.get_phase()[...] = ph
+    rows.finalize_init()
+    sp = Solver(np.array(lat, dtype=np.int32), 0, 0.1, layout="sparse")
+    sp.get_phase()[...] = ph
+    sp.finalize_init()
+    b_rows = rows.engine.info()["device_bytes"]
+    b_sp = sp.engine.info()["device_bytes"]
+    nfp = (nf + 63) // 64 * 64
+    ncell = 64 * 32 * 64
+    # rows: 19 f64 per cell; sparse: 19 f64 + 18 u32 per fluid site + 4 B per cell
+    assert b_rows - b_sp == 19 * 8 * ncell - (19 * 8 + 18 * 4 + 4) * nfp - 4 * ncell
+
+
+def test_sparse_rejects_adjoint():
+    from jyulb.isothermal import Solver
+    from jyulb import adjoint
+    from jyulb.defs import BGK
+    s = Solver(np.array([8, 4, 4], dtype=np.int32), BGK, 0.1, layout="sparse")
+    s.get_phase()[...] = 1
+    s.finalize_init()
+    with pytest.raises(ValueError, match="dense layouts"):
+        adjoint.flux_gradient(s, 4, 2)

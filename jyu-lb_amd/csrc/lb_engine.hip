@@ -212,6 +212,8 @@ struct StepParams {
     const uint32_t* cidx;
     const uint32_t* site;
     const uint32_t* tab;
+    const uint16_t* tab16;  // 16-bit form of tab (see step_sparse)
+    const uint32_t* tabw;   // its base locations, 18 per wave of 64 compact sites
     uint32_t nf;  // fluid sites = compact sites the step processes
 };
 
@@ -532,9 +534,14 @@ __global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
 // mirrored source; it writes f*_opp(l) back there. FULL sites may carry
 // moving-wall or Zou-He bits, read from the dense mask; lattices with
 // neither run the FULL = false kernel, which loads no mask.
-#ifndef LB_SP_RELOAD
-#define LB_SP_RELOAD 1
-#endif
+//
+// The step reads the table in a 16-bit form (k_sparse_pack): per wave of 64
+// compact sites and link, a 32-bit base location, and per site an offset
+// from it, SP_BOUNCE for the own slot l, or SP_FAR for a location out of the
+// offset's reach (boundary rows whose upstream row wraps, mirrored links),
+// which is then read from the 32-bit table.
+constexpr uint32_t SP_FAR = 0xFFFFu, SP_BOUNCE = 0xFFFEu;
+
 template <int P, int RLX, bool ODD, bool FULL>
 LBD void step_sparse(const StepParams& p, const uint32_t n) {
     using TS = typename Var<P>::TS;
@@ -558,8 +565,36 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l), p.inv_scale);
     } else {
+        // 16-bit codes: an offset from the wave's base for the link (scalar
+        // loads), own slot l (bounce-back), or the full 32-bit location. They
+        // are decoded with selects, so that all 18 population loads are in
+        // flight together; the rare SP_FAR links are fixed up in one
+        // wave-uniform branch.
+        const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * (Q - 1);
+        uint32_t base[Q - 1], code[Q];
 #pragma unroll
-        for (int l = 0; l < Q; ++l) loc[l] = l == REST ? REST * ps + n : p.tab[link_slot(l) * ps + n];
+        for (int sl = 0; sl < Q - 1; ++sl) base[sl] = B[sl];
+        // used here: the loads are not sunk into a branch per link
+#pragma unroll
+        for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]));
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            if (l != REST) code[l] = p.tab16[link_slot(l) * ps + n];
+        uint32_t far = 0;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            if (l == REST) {
+                loc[l] = REST * ps + n;
+                continue;
+            }
+            loc[l] = code[l] == SP_BOUNCE ? (uint32_t)l * ps + n : base[link_slot(l)] + code[l];
+            far |= (code[l] == SP_FAR ? 1u : 0u) << link_slot(l);
+        }
+        if (__ballot(far != 0)) {
+#pragma unroll
+            for (int l = 0; l < Q; ++l)
+                if (l != REST && ((far >> link_slot(l)) & 1u)) loc[l] = p.tab[link_slot(l) * ps + n];
+        }
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLO>(l == REST ? own(l) : F + loc[l], p.inv_scale);
     }
@@ -596,12 +631,6 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)), f[l], p.scale);
     } else {
-#if LB_SP_RELOAD
-        // the table entries again (cache hits): holding 18 locations across
-        // the collision would cost the FP64 kernel a wave per SIMD
-#pragma unroll
-        for (int l = 0; l < Q; ++l) loc[l] = l == REST ? 0u : __builtin_nontemporal_load(p.tab + link_slot(l) * ps + n);
-#endif
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSO>(l == REST ? own(l) : F + loc[l], f[opp(l)], p.scale);
     }
@@ -1147,6 +1176,8 @@ struct lb_solver {
     uint32_t* cidx = nullptr;
     uint32_t* site = nullptr;
     uint32_t* tab = nullptr;
+    uint16_t* tab16 = nullptr;
+    uint32_t* tabw = nullptr;
     uint32_t nfp = 0;
     uint8_t* phase = nullptr;
     double* den = nullptr;
@@ -1250,6 +1281,8 @@ static StepParams params(const lb_solver* s) {
         p.cidx = s->cidx;
         p.site = s->site;
         p.tab = s->tab;
+        p.tab16 = s->tab16;
+        p.tabw = s->tabw;
         p.nf = (uint32_t)s->nfluid;
     }
     return p;
@@ -1429,6 +1462,29 @@ __global__ void k_sparse_tab(StepParams q, const uint32_t* cidx, uint32_t nfp, u
         } else {
             tab[(int64_t)link_slot(l) * nfp + n] = (uint32_t)ls * nfp + ns;
         }
+    }
+}
+
+// 16-bit form of the odd-step table, one wave per 64 compact sites: per link
+// the smallest location that is not the site's own slot is the wave's base,
+// and every site stores its offset from it (see step_sparse).
+__global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_t nfp, uint32_t nf, uint16_t* tab16,
+                                                    uint32_t* tabw) {
+    const uint32_t w = blockIdx.x;
+    const uint32_t n = w * 64 + threadIdx.x;  // < nfp: the grid is nfp / 64 waves
+    for (int l = 0; l < Q; ++l) {
+        if (l == REST) continue;
+        const uint32_t sl = link_slot(l);
+        const uint32_t loc = tab[(size_t)sl * nfp + n];
+        const bool live = n < nf;
+        const bool bounce = live && loc == (uint32_t)l * nfp + n;
+        uint32_t b = (live && !bounce) ? loc : ~0u;
+        for (int o = 32; o > 0; o >>= 1) b = min(b, (uint32_t)__shfl_xor((int)b, o, 64));
+        uint32_t code = SP_FAR;
+        if (bounce) code = SP_BOUNCE;
+        else if (live && loc - b < SP_BOUNCE) code = loc - b;
+        tab16[(size_t)sl * nfp + n] = (uint16_t)code;
+        if (threadIdx.x == 0) tabw[(size_t)w * (Q - 1) + sl] = b == ~0u ? 0u : b;
     }
 }
 
@@ -1698,7 +1754,8 @@ int lb_destroy(lb_solver* s) {
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
                     s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
-                    s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab};
+                    s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab,
+                    s->tab16, s->tabw};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {
@@ -1798,9 +1855,13 @@ static int build_sparse(lb_solver* s) {
         dfree(s, &s->snap, Q * o * ts);
         dfree(s, (void**)&s->site, o * 4);
         dfree(s, (void**)&s->tab, (Q - 1) * o * 4);
+        dfree(s, (void**)&s->tab16, (Q - 1) * o * 2);
+        dfree(s, (void**)&s->tabw, (Q - 1) * (o / 64) * 4);
         s->nfp = 0;
         if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
-            (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)))
+            (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)) ||
+            (rc = dalloc(s, (void**)&s->tab16, (Q - 1) * nfp * 2)) ||
+            (rc = dalloc(s, (void**)&s->tabw, (Q - 1) * (nfp / 64) * 4)))
             return rc;
         s->nfp = (uint32_t)nfp;
     }
@@ -1848,6 +1909,10 @@ static int build_sparse(lb_solver* s) {
     HIPCHK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (herr) return fail(LB_EHIP, "sparse layout set-up: a link resolves to a non-fluid site");
+    k_sparse_pack<<<(unsigned)(nfp / 64), 64, 0, s->stream>>>(s->tab, (uint32_t)nfp, (uint32_t)nf, s->tab16,
+                                                             s->tabw);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
     return LB_OK;
 }
 

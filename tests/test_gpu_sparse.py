@@ -145,8 +145,11 @@ def test_sparse_device_memory_follows_fluid_sites():
     b_sp = sp.engine.info()["device_bytes"]
     nfp = (nf + 63) // 64 * 64
     ncell = 64 * 32 * 64
-    # rows: 19 f64 per cell; sparse: 19 f64 + 18 u32 per fluid site + 4 B per cell
-    assert b_rows - b_sp == 19 * 8 * ncell - (19 * 8 + 18 * 4 + 4) * nfp - 4 * ncell
+    # rows: 19 f64 per cell; sparse: per padded fluid site 19 f64, its site
+    # index, the 18 u32 locations and their 16-bit form, per wave of 64 the 18
+    # u32 bases, and a compact index per cell
+    sparse = (19 * 8 + 4 + 18 * 4 + 18 * 2) * nfp + 18 * 4 * (nfp // 64) + 4 * ncell
+    assert b_rows - b_sp == 19 * 8 * ncell - sparse
 
 
 def test_sparse_rejects_adjoint():

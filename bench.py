@@ -140,20 +140,47 @@ def porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02):
     return s, nfl
 
 
-def porous_medium(precision, lat=(768, 384, 384), seed=5, layout="rows"):
+def porous_medium(precision, lat=(768, 384, 384), seed=5, layout="rows", radius=(6, 12), solid=None):
     """Throughput workload: seeded random-sphere porous medium (radius 6..12,
     20 % solid), TRT tau = 0.6, gravity gx = 1e-6, periodic on all faces
-    (113M lattice cells)."""
+    (113M lattice cells). `solid` reuses a generated geometry."""
     from jyulb.isothermal import Solver
     from jyulb.geometry import porous_spheres
     from jyulb.defs import TRT, PERIODIC
     s = Solver(np.array(lat, dtype=np.int32), TRT, (0.6 - 0.5) / 3.0, np.array([1e-6, 0.0, 0.0]),
                np.full(6, PERIODIC, dtype=np.int32), precision=precision, layout=layout)
-    solid = porous_spheres(lat, 6, 12, 0.20, seed)
+    if solid is None:
+        solid = porous_spheres(lat, radius[0], radius[1], 0.20, seed)
     s.get_phase()[...] = np.where(solid, 0, 1)
-    del solid
     nfl = s.finalize_init()
     return s, nfl
+
+
+def sparse_workload(steps, precs, lat=(384, 192, 192)):
+    """Dense ROWS against the sparse fluid-only layout on a porous medium of
+    384x192x192 cells (spheres of radius 3..6 up to 20 % solid, seed 5: the
+    768x384x384 medium at half scale), TRT tau = 0.6, periodic."""
+    import torch
+    from jyulb.geometry import porous_spheres
+    solid = porous_spheres(lat, 3, 6, 0.20, 5)
+    ncell = int(np.prod(lat))
+    out = {}
+    for prec in precs:
+        row = {}
+        for layout in ("rows", "sparse"):
+            s, nfl = porous_medium(prec, lat=lat, layout=layout, solid=solid)
+            t, _ = time_device(s, steps, 10)
+            row[layout] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
+                               ms_per_step=round(1e3 * t / steps, 4))
+            del s
+            torch.cuda.synchronize()
+        row["sparse_over_rows"] = round(row["sparse"]["mlups"] / row["rows"]["mlups"], 3)
+        print(f"[bench] porous {lat[0]}x{lat[1]}x{lat[2]} {prec}: rows {row['rows']['mlups']} MLUPS, "
+              f"sparse {row['sparse']['mlups']} MLUPS", file=sys.stderr, flush=True)
+        out[prec] = row
+    return dict(workload=f"porous medium {lat[0]}x{lat[1]}x{lat[2]} (spheres r 3..6, 20% solid, seed 5), "
+                         "TRT tau=0.6, gx=1e-6, periodic; dense ROWS vs sparse fluid-only layout",
+                steps=steps, lattice_cells=ncell, fluid_sites=int(ncell - solid.sum()), variants=out)
 
 
 def throughput_workload(steps, precs, layouts=("rows",)):
@@ -304,6 +331,8 @@ def main():
                     help="steps of the 768x384x384 porous-medium throughput workload (0: skip, the default)")
     ap.add_argument("--porous768-layouts", default="rows",
                     help="population layouts of the porous-medium workload, e.g. rows,sparse")
+    ap.add_argument("--sparse-steps", type=int, default=200,
+                    help="steps of the rows-vs-sparse porous workload, 384x192x192 (0: skip)")
     ap.add_argument("--adj-interval", type=int, default=50)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain device copy of 2 GiB (torch copy_), the reference "
@@ -361,6 +390,7 @@ def main():
     duct_res = duct_precision(args.duct_steps, precs) if args.duct_steps > 0 else None
     big = (throughput_workload(args.porous768_steps, precs, args.porous768_layouts.split(","))
            if args.porous768_steps > 0 else None)
+    sparse = sparse_workload(args.sparse_steps, precs) if args.sparse_steps > 0 else None
     adj = None
     if args.adj_steps > 0:
         print(f"[bench] adjoint: {args.adj_steps} steps, checkpoint every {args.adj_interval}",
@@ -399,6 +429,7 @@ def main():
         "precision_error_vs_fp64": errors,
         "precision_workload": duct_res,
         "throughput_workload": big,
+        "sparse_workload": sparse,
         "cpu_baseline": cpu,
         "adjoint": adj,
     }

@@ -200,13 +200,13 @@ def throughput_workload(steps, precs, layouts=("rows",)):
                          achieved_gbs=round(BYTES_PER_SITE[prec] * nfl * steps / t / 1e9, 1),
                          device_bytes=int(s.engine.info()["device_bytes"]))
         del s
-    adj = None
-    if "fp32" in precs:
+    adj = {}
+    for layout in (layouts if "fp32" in precs else ()):
         # adjoint step cost at checkpoint interval 25 (50 steps: 3 checkpoints)
         import torch
         from jyulb.defs import FACE_X1, X
         from jyulb.adjoint import _prepare
-        s, nfl = porous_medium("fp32")
+        s, nfl = porous_medium("fp32", layout=layout)
         eng = s.engine
         eng.set_stream(torch.cuda.current_stream().cuda_stream)
         _prepare(s)
@@ -227,13 +227,16 @@ def throughput_workload(steps, precs, layouts=("rows",)):
             ev[3].record()
             torch.cuda.synchronize()
         t_f, t_af, t_ab = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]))
-        adj = dict(precision="fp32", steps=n_adj, checkpoint_interval=interval,
+        print(f"[bench] porous 768x384x384 adjoint fp32 {layout}: reverse/forward {t_ab / t_f:.3f}",
+              file=sys.stderr, flush=True)
+        adj["adjoint" if layout == "rows" else f"adjoint:{layout}"] = dict(
+                   precision="fp32", layout=layout, steps=n_adj, checkpoint_interval=interval,
                    forward_ms_per_step=round(t_f / n_adj, 4), adjoint_reverse_ms_per_step=round(t_ab / n_adj, 4),
                    adjoint_over_forward=round(t_ab / t_f, 3), gradient_over_forward=round((t_af + t_ab) / t_f, 3),
                    device_bytes=int(eng.info()["device_bytes"]))
         del s
     return dict(workload="porous medium 768x384x384 (spheres r 6..12, 20% solid), TRT tau=0.6, gx=1e-6, "
-                         "periodic", steps=steps, lattice_cells=ncell, variants=out, adjoint=adj)
+                         "periodic", steps=steps, lattice_cells=ncell, variants=out, **adj)
 
 
 def time_adjoint(precision, steps, interval):

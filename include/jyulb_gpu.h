@@ -48,8 +48,7 @@ extern "C" {
 #define LB_LAYOUT_SPARSE 3      /* fluid sites only: f[l][n] over the compacted fluid sites, with a table of
                                    the 18 odd-step locations per site (the fluid-only AA arrays of
                                    include/kernel/kernel.cpp:227-271), built on the device and read
-                                   as 8-bit codes from per-wave bases; allocated by lb_finalize;
-                                   no adjoint */
+                                   as 8-bit codes from per-wave bases; allocated by lb_finalize */
 
 typedef struct lb_solver lb_solver;
 

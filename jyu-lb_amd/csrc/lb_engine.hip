@@ -554,6 +554,40 @@ __global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
 // which is then read from the 32-bit table.
 constexpr uint32_t SP_FAR = (1u << (8 * sizeof(SpCode))) - 1u, SP_BOUNCE = SP_FAR - 1u;
 
+// Odd-step locations of the 18 links of compact site n (loc[REST]: own slot).
+// Codes: an offset from the wave's base for the link (scalar loads), own slot
+// l (bounce-back), or the full 32-bit location. They are decoded with
+// selects, so that all 18 population loads after this can be in flight
+// together; the rare SP_FAR links are fixed up in one wave-uniform branch.
+LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
+    const uint32_t ps = (uint32_t)p.ps;
+    const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * (Q - 1);
+    uint32_t base[Q - 1], code[Q];
+#pragma unroll
+    for (int sl = 0; sl < Q - 1; ++sl) base[sl] = B[sl];
+    // used here: the loads are not sunk into a branch per link
+#pragma unroll
+    for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]));
+#pragma unroll
+    for (int l = 0; l < Q; ++l)
+        if (l != REST) code[l] = p.tabc[link_slot(l) * ps + n];
+    uint32_t far = 0;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        if (l == REST) {
+            loc[l] = REST * ps + n;
+            continue;
+        }
+        loc[l] = code[l] == SP_BOUNCE ? (uint32_t)l * ps + n : base[link_slot(l)] + code[l];
+        far |= (code[l] == SP_FAR ? 1u : 0u) << link_slot(l);
+    }
+    if (__ballot(far != 0)) {
+#pragma unroll
+        for (int l = 0; l < Q; ++l)
+            if (l != REST && ((far >> link_slot(l)) & 1u)) loc[l] = p.tab[link_slot(l) * ps + n];
+    }
+}
+
 template <int P, int RLX, bool ODD, bool FULL>
 LBD void step_sparse(const StepParams& p, const uint32_t n) {
     using TS = typename Var<P>::TS;
@@ -577,36 +611,7 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l), p.inv_scale);
     } else {
-        // Codes: an offset from the wave's base for the link (scalar
-        // loads), own slot l (bounce-back), or the full 32-bit location. They
-        // are decoded with selects, so that all 18 population loads are in
-        // flight together; the rare SP_FAR links are fixed up in one
-        // wave-uniform branch.
-        const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * (Q - 1);
-        uint32_t base[Q - 1], code[Q];
-#pragma unroll
-        for (int sl = 0; sl < Q - 1; ++sl) base[sl] = B[sl];
-        // used here: the loads are not sunk into a branch per link
-#pragma unroll
-        for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]));
-#pragma unroll
-        for (int l = 0; l < Q; ++l)
-            if (l != REST) code[l] = p.tabc[link_slot(l) * ps + n];
-        uint32_t far = 0;
-#pragma unroll
-        for (int l = 0; l < Q; ++l) {
-            if (l == REST) {
-                loc[l] = REST * ps + n;
-                continue;
-            }
-            loc[l] = code[l] == SP_BOUNCE ? (uint32_t)l * ps + n : base[link_slot(l)] + code[l];
-            far |= (code[l] == SP_FAR ? 1u : 0u) << link_slot(l);
-        }
-        if (__ballot(far != 0)) {
-#pragma unroll
-            for (int l = 0; l < Q; ++l)
-                if (l != REST && ((far >> link_slot(l)) & 1u)) loc[l] = p.tab[link_slot(l) * ps + n];
-        }
+        sparse_locs(p, n, loc);
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLO>(l == REST ? own(l) : F + loc[l], p.inv_scale);
     }
@@ -677,6 +682,62 @@ struct AdjParams {
     double obj_scale;   // 1 / (fluid sites on the objective face)
 };
 
+// Per-site core of the adjoint step: the collision's vector-Jacobian product
+// from the recorded (rho, u*), the objective's seed, the acceleration
+// cotangent and, on Zou-He sites (zh), the faces' VJP in reverse order.
+// lam: cotangents of the post-collision populations; mu: of the streamed ones.
+template <int P, int RLX>
+LBD void adj_core(const StepParams& p, const AdjParams& x, const uint32_t c, const uint32_t m, const bool zh, int i,
+                  int j, int k, const typename Var<P>::TC* lam, typename Var<P>::TC* mu) {
+    using TC = typename Var<P>::TC;
+    const int64_t nq = p.nq;
+    const TC* R = (const TC*)x.rec;
+    const TC rho = R[c];
+    const TC us[3] = {R[nq + c], R[2 * nq + c], R[3 * nq + c]};
+    TC acc[3];
+    const TC* A = (const TC*)p.acc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
+    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
+    TC ga[3] = {TC(0), TC(0), TC(0)};
+    collide_vjp<RLX, TC>(lam, rho, us, acc, prm, mu, ga);
+    if (x.seed) {
+        // objective: mean over the face layer of rho u*_axis = j_axis + rho a_axis / 2
+        const int ax = x.obj_face >> 1;
+        const int pos = ax == 0 ? i : ax == 1 ? j : k;
+        const int n = ax == 0 ? p.d.nx : ax == 1 ? p.d.ny : p.d.nz;
+        if (pos == ((x.obj_face & 1) ? n - 1 : 0)) {
+            const TC sc = (TC)x.obj_scale;
+#pragma unroll
+            for (int l = 0; l < Q; ++l) mu[l] += sc * (TC(ccomp(l, x.obj_axis)) + TC(0.5) * acc[x.obj_axis]);
+            ga[x.obj_axis] += sc * TC(0.5) * rho;
+        }
+    }
+    if (x.gacc) {
+        TC* G = (TC*)x.gacc;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) G[(int64_t)a * nq + c] += ga[a];
+    }
+    if (zh) {
+        for (int fc = 5; fc >= 0; --fc) {
+            if (!(m & (M_ZH0 << fc))) continue;
+            int64_t fi;
+            if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
+            else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
+            else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
+            const TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
+            const int kind = p.d.bc[fc];
+            const TC dbc = kind == FIXED_DEN ? (TC)p.fden[fc][fi] : rho;
+            TC nu[Q], gv[3] = {TC(0), TC(0), TC(0)};
+            zh_vjp<TC>(fc, kind, dbc, vb, mu, nu, gv);
+#pragma unroll
+            for (int l = 0; l < Q; ++l) mu[l] = nu[l];
+            if (x.gface && fc == x.gface_id)
+                for (int a = 0; a < 3; ++a) x.gface[3 * fi + a] += (double)gv[a];
+        }
+    }
+}
+
 // One site of the adjoint step. MODE_PLAIN / MODE_BULK address the
 // cotangents like the bulk step kernel (wave-uniform row bases, bounce-back
 // links from the link mask); MODE_SPECIAL resolves every link.
@@ -714,51 +775,7 @@ LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, con
         else if constexpr (SPECIAL) lam[l] = L[e + (int64_t)opp(l) * p.ps];
         else lam[l] = ld_stream<NTE>(own(opp(l)) + kk);
     }
-    const TC* R = (const TC*)x.rec;
-    const TC rho = R[c];
-    const TC us[3] = {R[nq + c], R[2 * nq + c], R[3 * nq + c]};
-    TC acc[3];
-    const TC* A = (const TC*)p.acc;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
-    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
-    TC ga[3] = {TC(0), TC(0), TC(0)};
-    collide_vjp<RLX, TC>(lam, rho, us, acc, prm, mu, ga);
-    if (x.seed) {
-        // objective: mean over the face layer of rho u*_axis = j_axis + rho a_axis / 2
-        const int ax = x.obj_face >> 1;
-        const int pos = ax == 0 ? i : ax == 1 ? j : k;
-        const int n = ax == 0 ? p.d.nx : ax == 1 ? p.d.ny : p.d.nz;
-        if (pos == ((x.obj_face & 1) ? n - 1 : 0)) {
-            const TC sc = (TC)x.obj_scale;
-#pragma unroll
-            for (int l = 0; l < Q; ++l) mu[l] += sc * (TC(ccomp(l, x.obj_axis)) + TC(0.5) * acc[x.obj_axis]);
-            ga[x.obj_axis] += sc * TC(0.5) * rho;
-        }
-    }
-    if (x.gacc) {
-        TC* G = (TC*)x.gacc;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) G[(int64_t)a * nq + c] += ga[a];
-    }
-    if (SPECIAL && (m & M_ZH_ANY)) {
-        for (int fc = 5; fc >= 0; --fc) {
-            if (!(m & (M_ZH0 << fc))) continue;
-            int64_t fi;
-            if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
-            else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
-            else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
-            const TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
-            const int kind = p.d.bc[fc];
-            const TC dbc = kind == FIXED_DEN ? (TC)p.fden[fc][fi] : rho;
-            TC nu[Q], gv[3] = {TC(0), TC(0), TC(0)};
-            zh_vjp<TC>(fc, kind, dbc, vb, mu, nu, gv);
-#pragma unroll
-            for (int l = 0; l < Q; ++l) mu[l] = nu[l];
-            if (x.gface && fc == x.gface_id)
-                for (int a = 0; a < 3; ++a) x.gface[3 * fi + a] += (double)gv[a];
-        }
-    }
+    adj_core<P, RLX>(p, x, c, m, SPECIAL && (m & M_ZH_ANY), i, j, k, lam, mu);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         if constexpr (ODD) *loc(l) = mu[l];
@@ -791,6 +808,42 @@ __global__ __launch_bounds__(TILE) void k_adj_special(StepParams p, AdjParams x)
     if (t >= (ODD ? p.nspecial : p.nzh)) return;
     const uint32_t c = p.special[t];
     adj_site<P, RLX, ODD, MODE_SPECIAL>(p, x, c, p.mask[c]);
+}
+
+// Adjoint step of the sparse layout, one thread per compact fluid site: the
+// cotangents sit where the forward step's populations do (own slots on even
+// steps, the decoded odd-step locations on odd steps). ZH: the lattice has
+// Zou-He faces (their VJP is compiled in).
+template <int P, int RLX, bool ODD, bool ZH>
+__global__ __launch_bounds__(TILE) void k_adj_sp(StepParams p, AdjParams x) {
+    using TC = typename Var<P>::TC;
+    const uint32_t tile = tile_index(p);
+    if (tile >= p.ntiles) return;
+    const uint32_t n = tile * TILE + threadIdx.x;
+    if (n >= p.nf) return;
+    TC* L = (TC*)x.lam;
+    const uint32_t ps = (uint32_t)p.ps;
+    const uint32_t c = p.site[n];
+    const uint32_t m = ZH ? p.mask[c] : M_FLUID;
+    int i = 0, j = 0, k = 0;
+    if ((m & M_ZH_ANY) || x.seed) decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    constexpr bool NTE = (LB_NT_MASK & 1) != 0;
+    uint32_t loc[Q];
+    TC lam[Q], mu[Q];
+    if constexpr (ODD) {
+        sparse_locs(p, n, loc);
+#pragma unroll
+        for (int l = 0; l < Q; ++l) lam[opp(l)] = L[loc[l]];
+    } else {
+#pragma unroll
+        for (int l = 0; l < Q; ++l) lam[l] = ld_stream<NTE>(uniform_ptr(L + (int64_t)opp(l) * ps) + n);
+    }
+    adj_core<P, RLX>(p, x, c, m, ZH && (m & M_ZH_ANY) != 0, i, j, k, lam, mu);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        if constexpr (ODD) L[loc[l]] = mu[l];
+        else st_stream<NTE>(uniform_ptr(L + (int64_t)l * ps) + n, mu[l]);
+    }
 }
 
 // objective: sum over fluid sites of the face layer of rho u*_axis (FP64)
@@ -1626,6 +1679,16 @@ static void launch_any_step(const lb_solver* s, const StepParams& p, bool odd) {
 template <int P, int RLX>
 static void launch_adj(const lb_solver* s, const StepParams& p, const AdjParams& x, bool odd) {
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    if (p.tab) {  // sparse layout
+        if (p.nzh) {
+            if (odd) k_adj_sp<P, RLX, true, true><<<grid, TILE, 0, s->stream>>>(p, x);
+            else k_adj_sp<P, RLX, false, true><<<grid, TILE, 0, s->stream>>>(p, x);
+        } else {
+            if (odd) k_adj_sp<P, RLX, true, false><<<grid, TILE, 0, s->stream>>>(p, x);
+            else k_adj_sp<P, RLX, false, false><<<grid, TILE, 0, s->stream>>>(p, x);
+        }
+        return;
+    }
     if (odd) k_adj<P, RLX, true><<<grid, TILE, 0, s->stream>>>(p, x);
     else k_adj<P, RLX, false><<<grid, TILE, 0, s->stream>>>(p, x);
     const uint32_t ns = odd ? p.nspecial : p.nzh;
@@ -2114,7 +2177,6 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
     if (s->prec != LB_FP64 && s->prec != LB_FP32_SHIFTED)
         return fail(LB_EINVAL, "the adjoint runs in FP64 or FP32-shifted precision");
     if (s->moving) return fail(LB_EINVAL, "the adjoint supports stationary walls only (no moving solid sites)");
-    if (s->sparse) return fail(LB_EINVAL, "the adjoint runs on the dense layouts (not LB_LAYOUT_SPARSE)");
     const uint32_t nseg = (steps + interval - 1) / interval;
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
     int rc;

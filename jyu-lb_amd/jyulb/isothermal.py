@@ -20,7 +20,7 @@ Additions, keyword-only so that reference call sites are unaffected:
              arithmetic: the FP32 traffic with the rounding of storage only)
   layout     population layout on the device: "rows" (default), "soa",
              "soa_padded", or "sparse" (fluid sites only, with a table of the
-             odd-step locations: less traffic on porous media; no adjoint).
+             odd-step locations: less traffic on porous media).
              Results are identical in every layout.
 """
 import numpy as np

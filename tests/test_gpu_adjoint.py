@@ -16,14 +16,14 @@ from jyulb.defs import BGK, TRT, REGULARIZATION, WALL, PERIODIC, FIXED_DEN, FIXE
 from jyulb.defs import FACE_1X, FACE_X1, X
 
 
-def make_channel(rlx, precision, lat=(14, 9, 7), steps_bc=(WALL, SYMMETRIC), seed=3, solid=0.08):
+def make_channel(rlx, precision, lat=(14, 9, 7), steps_bc=(WALL, SYMMETRIC), seed=3, solid=0.08, layout="rows"):
     from jyulb.isothermal import Solver
     nx, ny, nz = lat
     bc = np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, steps_bc[0], steps_bc[1]], dtype=np.int32)
     if steps_bc[0] == PERIODIC:
         bc[4] = bc[5] = PERIODIC
     s = Solver(np.array(lat, dtype=np.int32), rlx, 0.2 / 3.0, np.array([1e-5, 0.0, 0.0]), bc,
-               precision=precision)
+               precision=precision, layout=layout)
     rng = np.random.default_rng(seed)
     ph = s.get_phase()
     ph[...] = 1
@@ -193,7 +193,7 @@ def test_adjoint_matches_torch_fp64_autograd(rlx):
     np.testing.assert_allclose(gf[1:-1, 1:-1], gvt[1:-1, 1:-1], rtol=0, atol=1e-9 * np.max(np.abs(gvt)))
 
 
-def make_porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02):
+def make_porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02, layout="rows"):
     """Adjoint workload: porous channel of seeded spheres (radius 4..8, 10 %
     solid, none in 8-layer buffers behind the inlet and outlet faces), BGK
     tau = 0.7, gravity gx = 1e-5, parabolic inlet (FIXED_VEL at 1X, peak
@@ -202,7 +202,8 @@ def make_porous_channel(precision, lat=(128, 64, 64), seed=4, peak=0.02):
     from jyulb.geometry import porous_spheres
     nx, ny, nz = lat
     s = Solver(np.array(lat, dtype=np.int32), BGK, 0.2 / 3.0, np.array([1e-5, 0.0, 0.0]),
-               np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, WALL, WALL], dtype=np.int32), precision=precision)
+               np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, WALL, WALL], dtype=np.int32), precision=precision,
+               layout=layout)
     solid = porous_spheres(lat, 4, 8, 0.10, seed, open_faces=(FACE_1X, FACE_X1), open_depth=8)
     s.get_phase()[...] = np.where(solid, 0, 1)
     prof = np.zeros((ny + 2, nz + 2))
@@ -255,3 +256,36 @@ def test_adjoint_identical_with_xcd_tile_order(monkeypatch):
     assert out[0][0] == out[1][0]
     np.testing.assert_array_equal(out[0][1], out[1][1])
     np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("rlx,steps_bc", [(BGK, (WALL, SYMMETRIC)), (TRT, (PERIODIC, PERIODIC)),
+                                          (REGULARIZATION, (WALL, SYMMETRIC))], ids=["bgk", "trt", "reg"])
+def test_adjoint_identical_in_sparse_layout(precision, rlx, steps_bc):
+    """The sparse layout's adjoint kernel reads and writes the cotangents at
+    the forward step's odd-step locations; objective and both gradients are
+    bit-identical to the dense layout's (Zou-He inlet/outlet, walls,
+    symmetric or periodic faces, seeded solids)."""
+    out = []
+    for layout in ("rows", "sparse"):
+        s, prof, amp = make_channel(rlx, precision, steps_bc=steps_bc, seed=8, layout=layout)
+        acc, fden, fvel, g = _inputs(s)
+        s.engine.snapshot()
+        out.append(_adjoint(s, acc, fden, fvel, g, 30, 7))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][2], out[0][2])
+
+
+def test_adjoint_porous_channel_sparse_equals_rows():
+    """Full-size porous channel (128x64x64, FP32, 60 steps, checkpoint every
+    25): the sparse layout's gradients equal the dense layout's bit for bit."""
+    out = []
+    for layout in ("rows", "sparse"):
+        s, prof, amp = make_porous_channel("fp32", layout=layout)
+        acc, fden, fvel, g = _inputs(s)
+        s.engine.snapshot()
+        out.append(_adjoint(s, acc, fden, fvel, g, 60, 25))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][2], out[0][2])

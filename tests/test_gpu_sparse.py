@@ -150,14 +150,3 @@ def test_sparse_device_memory_follows_fluid_sites():
     # u32 bases, and a compact index per cell
     sparse = (19 * 8 + 4 + 18 * 4 + 18) * nfp + 18 * 4 * (nfp // 64) + 4 * ncell
     assert b_rows - b_sp == 19 * 8 * ncell - sparse
-
-
-def test_sparse_rejects_adjoint():
-    from jyulb.isothermal import Solver
-    from jyulb import adjoint
-    from jyulb.defs import BGK
-    s = Solver(np.array([8, 4, 4], dtype=np.int32), BGK, 0.1, layout="sparse")
-    s.get_phase()[...] = 1
-    s.finalize_init()
-    with pytest.raises(ValueError, match="dense layouts"):
-        adjoint.flux_gradient(s, 4, 2)

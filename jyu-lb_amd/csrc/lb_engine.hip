@@ -210,6 +210,7 @@ struct StepParams {
     const uint8_t* wclass;  // per wave of 64 sites: 0 = plain (see k_mask)
     uint32_t xcd;           // deal tiles to XCDs in contiguous ranges (tile_index)
     uint32_t lds[2];        // dynamic LDS per workgroup of the bulk step launch (even, odd): caps occupancy
+    uint32_t tile;          // threads per workgroup of the launch that reads ntiles (see TILE)
     const uint32_t* mov_idx;  // per site: index of a moving site in ladd_u
     const double* ladd_u;     // per moving site: the 18 wall-velocity sums of Ladd's term
     uint32_t nwaves;
@@ -238,7 +239,11 @@ LBD int64_t eidx(const StepParams& p, uint32_t c) {
     return (int64_t)row * p.rs + (c - row * (uint32_t)p.d.nzp);
 }
 
+// Threads per workgroup: TILE for the adjoint and special-site kernels; the
+// bulk step kernels run p.tile threads (a multiple of 64, at most
+// STEP_TILE_MAX), chosen per precision at lb_create.
 constexpr int TILE = 256;
+constexpr int STEP_TILE_MAX = 512;
 constexpr int RED_BLOCK_ADJ = 256;
 
 // element offset of the location an odd step reads f_l from (and writes
@@ -498,7 +503,7 @@ template <int P, int RLX, bool ODD>
 LBD void step_bulk(const StepParams& p) {
     const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
-    const uint32_t c = tile * TILE + threadIdx.x;
+    const uint32_t c = tile * p.tile + threadIdx.x;
     // nq is a multiple of 64, so a wave is entirely inside or outside [0, nq)
     const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
     if (w >= p.nwaves) return;
@@ -512,7 +517,7 @@ LBD void step_bulk(const StepParams& p) {
 }
 
 template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) LB_OCC void k_step(StepParams p) {
+__global__ __launch_bounds__(STEP_TILE_MAX) LB_OCC void k_step(StepParams p) {
     step_bulk<P, RLX, ODD>(p);
 }
 
@@ -521,7 +526,7 @@ __global__ __launch_bounds__(TILE) LB_OCC void k_step(StepParams p) {
 // the TRT cavity). BGK and the regularized scheme measured 9-10% slower with
 // the hint (128x64x64 porous channel, 512x128x128 duct) and keep k_step.
 template <int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_h(StepParams p) {
+__global__ __launch_bounds__(STEP_TILE_MAX) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_h(StepParams p) {
     step_bulk<LB_FP16_STORAGE, RLX, ODD>(p);
 }
 
@@ -654,10 +659,10 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
 }
 
 template <int P, int RLX, bool ODD, bool FULL>
-__global__ __launch_bounds__(TILE) void k_step_sp(StepParams p) {
+__global__ __launch_bounds__(STEP_TILE_MAX) void k_step_sp(StepParams p) {
     const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
-    const uint32_t n = tile * TILE + threadIdx.x;
+    const uint32_t n = tile * p.tile + threadIdx.x;
     if (n >= p.nf) return;
     step_sparse<P, RLX, ODD, FULL>(p, n);
 }
@@ -789,7 +794,7 @@ template <int P, int RLX, bool ODD>
 __global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
     const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
-    const uint32_t c = tile * TILE + threadIdx.x;
+    const uint32_t c = tile * p.tile + threadIdx.x;
     const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
     if (w >= p.nwaves) return;
     if (LB_PLAIN_PATH && p.wclass[w] == 0) {
@@ -819,7 +824,7 @@ __global__ __launch_bounds__(TILE) void k_adj_sp(StepParams p, AdjParams x) {
     using TC = typename Var<P>::TC;
     const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
-    const uint32_t n = tile * TILE + threadIdx.x;
+    const uint32_t n = tile * p.tile + threadIdx.x;
     if (n >= p.nf) return;
     TC* L = (TC*)x.lam;
     const uint32_t ps = (uint32_t)p.ps;
@@ -1226,6 +1231,7 @@ struct lb_solver {
     hipStream_t stream = nullptr;
     uint64_t xcd_cells = LB_XCD_REMAP_CELLS;  // tile order threshold (tile_index)
     uint32_t step_lds[2] = {0, 0};            // see StepParams::lds
+    uint32_t step_tile = TILE;                // see StepParams::tile
     uint32_t* mov_idx = nullptr;  // moving-site index per site (k_ladd_prep)
     double* ladd_u = nullptr;     // 18 Ladd velocity sums per moving site
     size_t ladd_cap = 0;          // moving sites ladd_u has room for
@@ -1327,7 +1333,8 @@ static StepParams params(const lb_solver* s) {
     p.NJ = s->NJ;
     p.NK = s->NK;
     p.nq = s->nq;
-    p.ntiles = ((s->sparse ? s->nfp : s->nq) + TILE - 1) / TILE;
+    p.tile = s->step_tile;
+    p.ntiles = ((s->sparse ? s->nfp : s->nq) + p.tile - 1) / p.tile;
     p.xcd = s->nq >= s->xcd_cells ? 1u : 0u;
     p.lds[0] = s->step_lds[0];
     p.lds[1] = s->step_lds[1];
@@ -1637,20 +1644,20 @@ static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
     if (p.tab) {  // sparse layout: one kernel, with the mask only where needed
         if (p.moving || p.nzh) {
-            if (odd) k_step_sp<P, RLX, true, true><<<grid, TILE, 0, st>>>(p);
-            else k_step_sp<P, RLX, false, true><<<grid, TILE, 0, st>>>(p);
+            if (odd) k_step_sp<P, RLX, true, true><<<grid, p.tile, 0, st>>>(p);
+            else k_step_sp<P, RLX, false, true><<<grid, p.tile, 0, st>>>(p);
         } else {
-            if (odd) k_step_sp<P, RLX, true, false><<<grid, TILE, 0, st>>>(p);
-            else k_step_sp<P, RLX, false, false><<<grid, TILE, 0, st>>>(p);
+            if (odd) k_step_sp<P, RLX, true, false><<<grid, p.tile, 0, st>>>(p);
+            else k_step_sp<P, RLX, false, false><<<grid, p.tile, 0, st>>>(p);
         }
         return;
     }
     if constexpr (P == LB_FP16_STORAGE && RLX == TRT) {
-        if (odd) k_step_h<RLX, true><<<grid, TILE, p.lds[1], st>>>(p);
-        else k_step_h<RLX, false><<<grid, TILE, p.lds[0], st>>>(p);
+        if (odd) k_step_h<RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
+        else k_step_h<RLX, false><<<grid, p.tile, p.lds[0], st>>>(p);
     } else {
-        if (odd) k_step<P, RLX, true><<<grid, TILE, p.lds[1], st>>>(p);
-        else k_step<P, RLX, false><<<grid, TILE, p.lds[0], st>>>(p);
+        if (odd) k_step<P, RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
+        else k_step<P, RLX, false><<<grid, p.tile, p.lds[0], st>>>(p);
     }
     const uint32_t ns = odd ? p.nspecial : p.nzh;
     if (ns) {
@@ -1677,7 +1684,10 @@ static void launch_any_step(const lb_solver* s, const StepParams& p, bool odd) {
 }
 
 template <int P, int RLX>
-static void launch_adj(const lb_solver* s, const StepParams& p, const AdjParams& x, bool odd) {
+static void launch_adj(const lb_solver* s, const StepParams& ps, const AdjParams& x, bool odd) {
+    StepParams p = ps;  // the adjoint kernels run TILE threads per workgroup
+    p.tile = TILE;
+    p.ntiles = ((p.tab ? (uint32_t)p.ps : p.nq) + TILE - 1) / TILE;
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
     if (p.tab) {  // sparse layout
         if (p.nzh) {
@@ -1747,6 +1757,17 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     if (const char* env = getenv("LB_STEP_LDS_EVEN")) s->step_lds[0] = (uint32_t)strtoul(env, nullptr, 10);
     if (const char* env = getenv("LB_STEP_LDS_ODD")) s->step_lds[1] = (uint32_t)strtoul(env, nullptr, 10);
     for (uint32_t& b : s->step_lds) b = b > 65536u ? 65536u : b;  // 64 KiB keeps one workgroup per CU
+    // Bulk step workgroups: the dense FP64 kernel (4 waves per SIMD) measured
+    // 4% faster with 512 threads on the 256^3 cavity (1.3% slower on the
+    // 512x128x128 duct, even on porous media); FP32 storage measured 8% slower
+    // with 512 on the cavity, and the sparse FP64 odd step holds 3 waves per
+    // SIMD, of which a 512-thread workgroup would leave 2: those and the
+    // rest keep 256. LB_STEP_TILE (a multiple of 64 up to 512) overrides it.
+    if (s->prec == LB_FP64 && cfg->layout != LB_LAYOUT_SPARSE) s->step_tile = 512;
+    if (const char* env = getenv("LB_STEP_TILE")) {
+        const unsigned long t = strtoul(env, nullptr, 10);
+        if (t >= 64 && t <= (unsigned long)STEP_TILE_MAX && t % 64 == 0) s->step_tile = (uint32_t)t;
+    }
     s->rho0 = cfg->rho0 > 0 ? cfg->rho0 : 1.0;
     s->scale = cfg->fp16_scale > 0 ? cfg->fp16_scale : 1.0;
     // relaxation parameters, isothermal.pyx:273-287

@@ -342,8 +342,9 @@ def main():
                          "kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not supported "
-              "(the path's slab decomposition is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py: runs on one GPU only; multi-GPU runs are not supported (the path would "
+              "shard as 1-D slabs along x, one process per GPU, exchanging the 5 populations that "
+              "cross each slab face per step).", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

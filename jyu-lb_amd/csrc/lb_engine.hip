@@ -305,6 +305,11 @@ LBD void ladd_apply(const StepParams& p, uint32_t c, uint32_t m, T* f) {
     }
 }
 
+// Build switches of the bulk kernels. LB_PLAIN_PATH=1: waves whose 64 sites
+// are all plain fluid (wave class 0) take the path that loads no link mask;
+// 0 sends every wave through the mask path. LB_WAVES_PER_EU > 0 asks the
+// compiler for that many waves per SIMD in k_step (0: its own choice; 5 or 6
+// for the FP64 kernels spill registers and measured about half the speed).
 #ifndef LB_PLAIN_PATH
 #define LB_PLAIN_PATH 1
 #endif

@@ -1649,11 +1649,11 @@ static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
     if (p.tab) {  // sparse layout: one kernel, with the mask only where needed
         if (p.moving || p.nzh) {
-            if (odd) k_step_sp<P, RLX, true, true><<<grid, p.tile, 0, st>>>(p);
-            else k_step_sp<P, RLX, false, true><<<grid, p.tile, 0, st>>>(p);
+            if (odd) k_step_sp<P, RLX, true, true><<<grid, p.tile, p.lds[1], st>>>(p);
+            else k_step_sp<P, RLX, false, true><<<grid, p.tile, p.lds[0], st>>>(p);
         } else {
-            if (odd) k_step_sp<P, RLX, true, false><<<grid, p.tile, 0, st>>>(p);
-            else k_step_sp<P, RLX, false, false><<<grid, p.tile, 0, st>>>(p);
+            if (odd) k_step_sp<P, RLX, true, false><<<grid, p.tile, p.lds[1], st>>>(p);
+            else k_step_sp<P, RLX, false, false><<<grid, p.tile, p.lds[0], st>>>(p);
         }
         return;
     }
@@ -1753,12 +1753,17 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     s->nq = (uint32_t)((int64_t)s->d.nx * s->d.ny * s->d.nzp);
     s->ncell = (int64_t)s->d.nx * s->d.ny * s->d.nz;
     s->prec = cfg->precision;
+    // The sparse FP32-storage kernels measured 5% faster on the 768x384x384
+    // medium in the plain round-robin order (the sparse FP64 kernel 4% faster
+    // with the XCD order, FP16 the same either way).
+    if (cfg->layout == LB_LAYOUT_SPARSE && (s->prec == LB_FP32_SHIFTED || s->prec == LB_FP32_STORAGE))
+        s->xcd_cells = UINT64_MAX;
     if (const char* env = getenv("LB_XCD_REMAP_CELLS")) s->xcd_cells = strtoull(env, nullptr, 10);
     // Unused dynamic LDS caps the bulk step's occupancy. The FP32 kernel needs
     // few registers (8 waves per SIMD); its even step streamed 3-4% faster at
     // 6 waves per SIMD (26 KB per workgroup of 256 threads), while capping the
     // other variants or the odd step measured slower.
-    if (s->prec == LB_FP32_SHIFTED) s->step_lds[0] = 26000;
+    if (s->prec == LB_FP32_SHIFTED && cfg->layout != LB_LAYOUT_SPARSE) s->step_lds[0] = 26000;
     if (const char* env = getenv("LB_STEP_LDS_EVEN")) s->step_lds[0] = (uint32_t)strtoul(env, nullptr, 10);
     if (const char* env = getenv("LB_STEP_LDS_ODD")) s->step_lds[1] = (uint32_t)strtoul(env, nullptr, 10);
     for (uint32_t& b : s->step_lds) b = b > 65536u ? 65536u : b;  // 64 KiB keeps one workgroup per CU

@@ -126,6 +126,18 @@ def test_gpu_fp64_bit_exact_with_xcd_tile_order(name, layout, monkeypatch):
     test_gpu_fp64_bit_exact(path, layout)
 
 
+@pytest.mark.parametrize("tile", ["128", "256", "512"])
+@pytest.mark.parametrize("name", ["trt_walls", "reg_zouhe_y", "bgk_symmetric"])
+def test_gpu_fp64_bit_exact_with_step_tile(name, tile, monkeypatch):
+    """The bulk step's workgroup size is chosen per solver (512 threads for
+    the dense FP64 kernel); any size gives the same bits, in the dense and
+    the sparse layout."""
+    monkeypatch.setenv("LB_STEP_TILE", tile)
+    path = os.path.join(os.path.dirname(__file__), "golden", name + ".npz")
+    test_gpu_fp64_bit_exact(path, 0)
+    test_gpu_fp64_bit_exact(path, 3)
+
+
 # Reduced-precision variants on every reference scenario: all boundary types
 # (periodic, walls, symmetric, Zou-He on each face, moving walls, mutations
 # between evolve calls) through the shifted-population forms. The density

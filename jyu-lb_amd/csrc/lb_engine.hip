@@ -1237,6 +1237,8 @@ struct lb_solver {
     uint64_t xcd_cells = LB_XCD_REMAP_CELLS;  // tile order threshold (tile_index)
     uint32_t step_lds[2] = {0, 0};            // see StepParams::lds
     uint32_t step_tile = TILE;                // see StepParams::tile
+    bool tune = false;                        // choose the step launch on the next long evolve (tune_step)
+    bool tune_verbose = false;
     uint32_t* mov_idx = nullptr;  // moving-site index per site (k_ladd_prep)
     double* ladd_u = nullptr;     // 18 Ladd velocity sums per moving site
     size_t ladd_cap = 0;          // moving sites ladd_u has room for
@@ -1778,6 +1780,20 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
         const unsigned long t = strtoul(env, nullptr, 10);
         if (t >= 64 && t <= (unsigned long)STEP_TILE_MAX && t % 64 == 0) s->step_tile = (uint32_t)t;
     }
+    // Launch tuning on the first long evolve call, for lattices of at least
+    // 2^22 cells (below that launch overhead dominates the timings), unless
+    // the launch is pinned by LB_STEP_TILE / LB_STEP_LDS_* or LB_AUTOTUNE=0.
+    // LB_AUTOTUNE_MIN_CELLS changes the size, LB_AUTOTUNE_VERBOSE=1 prints
+    // the timings.
+    {
+        uint64_t min_cells = 1ull << 22;
+        if (const char* env = getenv("LB_AUTOTUNE_MIN_CELLS")) min_cells = strtoull(env, nullptr, 10);
+        const char* at = getenv("LB_AUTOTUNE");
+        s->tune = (!at || strtol(at, nullptr, 10) != 0) && !getenv("LB_STEP_TILE") && !getenv("LB_STEP_LDS_EVEN") &&
+                  !getenv("LB_STEP_LDS_ODD") && (uint64_t)s->nq >= min_cells;
+        const char* vb = getenv("LB_AUTOTUNE_VERBOSE");
+        s->tune_verbose = vb && strtol(vb, nullptr, 10) != 0;
+    }
     s->rho0 = cfg->rho0 > 0 ? cfg->rho0 : 1.0;
     s->scale = cfg->fp16_scale > 0 ? cfg->fp16_scale : 1.0;
     // relaxation parameters, isothermal.pyx:273-287
@@ -2074,23 +2090,113 @@ int lb_refresh_walls(lb_solver* s) {
     return LB_OK;
 }
 
+// `n` steps with the solver's current launch configuration; the last one
+// stores den/vel if `store_last`.
+static void run_steps(lb_solver* s, uint32_t n, bool store_last) {
+    StepParams p = params(s);
+    for (uint32_t t = 0; t < n; ++t) {
+        p.store_dv = (store_last && t + 1 == n) ? 1 : 0;
+        launch_any_step(s, p, s->next_odd != 0);
+        s->next_odd ^= 1;
+        s->steps++;
+    }
+}
+
+// Launch configurations of the bulk step that the first long evolve call
+// chooses between (tune_step). The best one depends on the workload: for
+// FP64, 4 waves per SIMD in 512-thread workgroups were best on the 256^3
+// cavity and 3 waves (256 threads, LDS-capped) 5% faster on the 512x128x128
+// duct. Every configuration gives bit-identical results, so the candidates
+// are timed on the call's own steps.
+struct StepCfg {
+    uint32_t tile, lds_even, lds_odd;
+};
+constexpr int TUNE_WARM = 2, TUNE_STEPS = 4;
+
+static int tune_candidates(const lb_solver* s, StepCfg* c) {
+    int n = 0;
+    c[n++] = {s->step_tile, s->step_lds[0], s->step_lds[1]};  // the default
+    auto add = [&](StepCfg x) {
+        for (int i = 0; i < n; ++i)
+            if (c[i].tile == x.tile && c[i].lds_even == x.lds_even && c[i].lds_odd == x.lds_odd) return;
+        c[n++] = x;
+    };
+    if (s->sparse) return n;
+    if (s->prec == LB_FP64 || s->prec == LB_FP32_STORAGE) {
+        add({512, 0, 0});
+        add({256, 0, 0});
+        add({256, 40000, 40000});  // 4 workgroups per CU: 4 waves per SIMD
+        add({256, 50000, 50000});  // 3 waves per SIMD
+    } else if (s->prec == LB_FP32_SHIFTED) {
+        add({256, 0, 0});
+        add({256, 26000, 0});
+    }
+    return n;
+}
+
+static int tune_step(lb_solver* s) {
+    StepCfg c[6];
+    const int n = tune_candidates(s, c);
+    s->tune = false;
+    if (n < 2) return LB_OK;
+    std::vector<hipEvent_t> ev(2 * n, nullptr);
+    for (hipEvent_t& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    for (hipEvent_t e : ev)
+        if (!e) {
+            for (hipEvent_t x : ev)
+                if (x) (void)hipEventDestroy(x);
+            return LB_OK;  // no timing: keep the default
+        }
+    run_steps(s, TUNE_WARM, false);
+    for (int i = 0; i < n; ++i) {
+        s->step_tile = c[i].tile;
+        s->step_lds[0] = c[i].lds_even;
+        s->step_lds[1] = c[i].lds_odd;
+        HIPCHK(hipEventRecord(ev[2 * i], s->stream));
+        run_steps(s, TUNE_STEPS, false);
+        HIPCHK(hipEventRecord(ev[2 * i + 1], s->stream));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(ev[2 * n - 1]));
+    int best = 0;
+    float tbest = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        if (i == 0 || ms < tbest) {
+            best = i;
+            tbest = ms;
+        }
+        if (s->tune_verbose)
+            fprintf(stderr, "[jyulb] step launch tile %u, LDS %u/%u: %.4f ms per step\n", c[i].tile, c[i].lds_even,
+                    c[i].lds_odd, ms / TUNE_STEPS);
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    s->step_tile = c[best].tile;
+    s->step_lds[0] = c[best].lds_even;
+    s->step_lds[1] = c[best].lds_odd;
+    return LB_OK;
+}
+
 int lb_evolve(lb_solver* s, uint32_t steps) {
     if (!s) return fail(LB_EINVAL, "null handle");
     if (!s->finalized) return fail(LB_ESTATE, "Finalize solver initialization before calling evolve");
     if (s->nfluid < 1 || steps == 0) return LB_OK;
-    StepParams p = params(s);
-    for (uint32_t t = 0; t < steps; ++t) {
-        p.store_dv = (t + 1 == steps) ? 1 : 0;
-        const bool odd = s->next_odd != 0;
-        switch (s->prec) {
-        case LB_FP64: launch_step_rlx<LB_FP64>(p, s->cfg.rlx_scheme, odd, s->stream); break;
-        case LB_FP32_SHIFTED: launch_step_rlx<LB_FP32_SHIFTED>(p, s->cfg.rlx_scheme, odd, s->stream); break;
-        case LB_FP32_STORAGE: launch_step_rlx<LB_FP32_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
-        default: launch_step_rlx<LB_FP16_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
+    if (s->tune) {
+        // tuning takes TUNE_WARM + TUNE_STEPS per candidate of this call's
+        // steps; a shorter call leaves it to a later one
+        StepCfg c[6];
+        const int n = tune_candidates(s, c);
+        if (n < 2) s->tune = false;
+        else if (steps > (uint32_t)(TUNE_WARM + n * TUNE_STEPS)) {
+            const uint64_t before = s->steps;
+            int rc = tune_step(s);
+            if (rc) return rc;
+            steps -= (uint32_t)(s->steps - before);
         }
-        s->next_odd ^= 1;
-        s->steps++;
     }
+    run_steps(s, steps, true);
     HIPCHK(hipGetLastError());
     return LB_OK;
 }

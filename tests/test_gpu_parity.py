@@ -138,6 +138,32 @@ def test_gpu_fp64_bit_exact_with_step_tile(name, tile, monkeypatch):
     test_gpu_fp64_bit_exact(path, 3)
 
 
+@pytest.mark.parametrize("name", ["trt_long", "ex_couette", "ex_poiseuille_grav_sym", "c1_poiseuille"])
+def test_gpu_fp64_bit_exact_with_launch_tuning(name, monkeypatch):
+    """The first long evolve call times its own steps under each candidate
+    launch configuration and keeps the fastest; forced on for these small
+    lattices, the FP64 results still equal the reference's bit for bit."""
+    monkeypatch.setenv("LB_AUTOTUNE", "1")
+    monkeypatch.setenv("LB_AUTOTUNE_MIN_CELLS", "0")
+    path = os.path.join(os.path.dirname(__file__), "golden", name + ".npz")
+    test_gpu_fp64_bit_exact(path, 0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32_storage", "fp16"])
+@pytest.mark.parametrize("name", ["trt_long", "ex_couette"])
+def test_launch_tuning_changes_no_bit(name, precision, monkeypatch):
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
+    sc = json.loads(str(z["scenario"]))
+    Solver = _solver_cls()
+    monkeypatch.setenv("LB_AUTOTUNE_MIN_CELLS", "0")
+    outs = []
+    for env in ("0", "1"):
+        monkeypatch.setenv("LB_AUTOTUNE", env)
+        outs.append(scenarios.run_scenario(lambda *a: Solver(*a, precision=precision), sc, record_f=True))
+    for k in outs[0]:
+        np.testing.assert_array_equal(outs[1][k], outs[0][k], err_msg=f"{name} {precision} {k}")
+
+
 # Reduced-precision variants on every reference scenario: all boundary types
 # (periodic, walls, symmetric, Zou-He on each face, moving walls, mutations
 # between evolve calls) through the shifted-population forms. The density

@@ -144,11 +144,18 @@ int lb_set_face_data_device(lb_solver *s, int face, const double *den_dev, const
  * through that layer, e.g. face X1, axis 0 = mean outlet x-flux). FP64. */
 int lb_objective(lb_solver *s, int face, int axis, double *J);
 
+/* Objective selector of lb_adj_forward: instead of a face flux, the mean over
+ * the fluid sites of Pi_neq:Pi_neq, the squared non-equilibrium momentum flux
+ * (proportional to the viscous dissipation) of the last step's collision
+ * input; `axis` is ignored. Dense layouts. */
+#define LB_OBJ_DISSIPATION 6
+
 /* Adjoint, forward pass (no counterpart in the reference: reverse-mode
  * differentiation of `steps` evolve steps with fixed parameters). Runs the
  * steps like lb_evolve, keeping a checkpoint of the population array every
- * `interval` steps, and returns the objective J (lb_objective) after the last
- * step. FP64 or FP32 precision; stationary walls only. */
+ * `interval` steps, and returns the objective J: lb_objective(face, axis)
+ * after the last step, or with face = LB_OBJ_DISSIPATION the dissipation
+ * objective. FP64 or FP32 precision; stationary walls only. */
 int lb_adj_forward(lb_solver *s, uint32_t steps, uint32_t interval, int face, int axis, double *J);
 
 /* Adjoint, reverse pass: gradients of J from the last lb_adj_forward with

@@ -228,6 +228,11 @@ struct StepParams {
     const SpCode* tabc;     // compact form of tab (see step_sparse)
     const uint32_t* tabw;   // its base locations, 18 per wave of 64 compact sites
     uint32_t nf;  // fluid sites = compact sites the step processes
+    // dissipation objective (k_diss): per site 19 population seeds, 3
+    // acceleration seeds and the site's Pi^neq:Pi^neq, each an nq-long row of
+    // the compute type; dscale = 1 / (fluid sites)
+    void* dseed;
+    double dscale;
 };
 
 // expanded index of padded site c in the population array: population l of
@@ -350,7 +355,65 @@ LBD void zouhe_site(const StepParams& p, uint32_t m, int i, int j, int k, TC rho
     }
 }
 
-template <int P, int RLX, bool ODD, int MODE>
+// Dissipation objective J = (1/N) sum over the fluid sites of Pi^neq:Pi^neq,
+// the non-equilibrium momentum flux of the final step's collision input f
+// (streamed, bounce-back and Zou-He applied): Pi^neq = sum_l c_l c_l f_l -
+// rho u* u* - rho/3 I with u* = j/rho + a/2. Writes the site's term and the
+// seeds dJ/df_l = (2/N) [c_l.Pi.c_l - 2 (c_l + a/2).Pi.u* + u*.Pi.u* -
+// tr(Pi)/3] and dJ/da = -(2/N) rho Pi.u* (Pi = Pi^neq, symmetric).
+template <class TC, bool SH>
+LBD void diss_site(const StepParams& p, const uint32_t c, const TC* f, const TC* acc) {
+    TC s0 = TC(0), j[3] = {TC(0), TC(0), TC(0)}, Pi[6] = {TC(0), TC(0), TC(0), TC(0), TC(0), TC(0)};
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const TC cx = TC(CI[l]), cy = TC(CJ[l]), cz = TC(CK[l]);
+        s0 += f[l];
+        j[0] += cx * f[l];
+        j[1] += cy * f[l];
+        j[2] += cz * f[l];
+        Pi[0] += cx * cx * f[l];
+        Pi[1] += cx * cy * f[l];
+        Pi[2] += cx * cz * f[l];
+        Pi[3] += cy * cy * f[l];
+        Pi[4] += cy * cz * f[l];
+        Pi[5] += cz * cz * f[l];
+    }
+    const TC rho0 = (TC)p.rho0, third = TC(1.0) / TC(3.0);
+    const TC rho = SH ? rho0 + s0 : s0;
+    if (SH) {  // shifted populations: sum_l w_l c_l c_l rho0 = rho0/3 I
+        Pi[0] += rho0 * third;
+        Pi[3] += rho0 * third;
+        Pi[5] += rho0 * third;
+    }
+    TC u[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) u[a] = j[a] / rho + TC(0.5) * acc[a];
+    const TC N[6] = {Pi[0] - rho * u[0] * u[0] - rho * third, Pi[1] - rho * u[0] * u[1], Pi[2] - rho * u[0] * u[2],
+                     Pi[3] - rho * u[1] * u[1] - rho * third, Pi[4] - rho * u[1] * u[2],
+                     Pi[5] - rho * u[2] * u[2] - rho * third};
+    const TC Nu[3] = {N[0] * u[0] + N[1] * u[1] + N[2] * u[2], N[1] * u[0] + N[3] * u[1] + N[4] * u[2],
+                      N[2] * u[0] + N[4] * u[1] + N[5] * u[2]};
+    const TC uNu = u[0] * Nu[0] + u[1] * Nu[1] + u[2] * Nu[2];
+    const TC tr = N[0] + N[3] + N[5];
+    const TC sc = TC(2.0) * (TC)p.dscale;
+    TC* S = (TC*)p.dseed;
+    const int64_t nq = p.nq;
+#pragma unroll
+    for (int l = 0; l < Q; ++l) {
+        const TC cx = TC(CI[l]), cy = TC(CJ[l]), cz = TC(CK[l]);
+        const TC cNc = cx * (N[0] * cx + N[1] * cy + N[2] * cz) + cy * (N[1] * cx + N[3] * cy + N[4] * cz) +
+                       cz * (N[2] * cx + N[4] * cy + N[5] * cz);
+        const TC wNu =
+            (cx + TC(0.5) * acc[0]) * Nu[0] + (cy + TC(0.5) * acc[1]) * Nu[1] + (cz + TC(0.5) * acc[2]) * Nu[2];
+        S[(int64_t)l * nq + c] = sc * (cNc - TC(2.0) * wNu + uNu - tr * third);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) S[(int64_t)(Q + a) * nq + c] = -sc * rho * Nu[a];
+    S[(int64_t)(Q + 3) * nq + c] =
+        N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + TC(2.0) * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
+}
+
+template <int P, int RLX, bool ODD, int MODE, bool DISS = false>
 LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     constexpr bool PLAIN = MODE != MODE_SPECIAL;  // no special handling needed
     using TS = typename Var<P>::TS;
@@ -431,6 +494,10 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     const TC* A = (const TC*)p.acc;
 #pragma unroll
     for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
+    if constexpr (DISS) {  // objective gather: nothing is stored to f
+        diss_site<TC, SH>(p, c, f, acc);
+        return;
+    }
     TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
     TC vel[3];
     const TC den = collide<RLX, TC, SH>(f, acc, prm, rho0, vel);
@@ -504,7 +571,7 @@ template <bool ODD>
 LBD constexpr uint32_t special_bits() { return ODD ? M_SPECIAL : M_ZH_ANY; }
 
 // Bulk step: every fluid site that is not special for this parity.
-template <int P, int RLX, bool ODD>
+template <int P, int RLX, bool ODD, bool DISS = false>
 LBD void step_bulk(const StepParams& p) {
     const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
@@ -513,12 +580,12 @@ LBD void step_bulk(const StepParams& p) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
     if (w >= p.nwaves) return;
     if (LB_PLAIN_PATH && p.wclass[w] == 0) {
-        step_site<P, RLX, ODD, MODE_PLAIN>(p, c, M_FLUID);
+        step_site<P, RLX, ODD, MODE_PLAIN, DISS>(p, c, M_FLUID);
         return;
     }
     const uint32_t m = p.mask[c];
     if ((m & (M_FLUID | special_bits<ODD>())) != M_FLUID) return;
-    step_site<P, RLX, ODD, MODE_BULK>(p, c, m);
+    step_site<P, RLX, ODD, MODE_BULK, DISS>(p, c, m);
 }
 
 template <int P, int RLX, bool ODD>
@@ -542,6 +609,21 @@ __global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
     if (t >= (ODD ? p.nspecial : p.nzh)) return;
     const uint32_t c = p.special[t];
     step_site<P, RLX, ODD, MODE_SPECIAL>(p, c, p.mask[c]);
+}
+
+// Dissipation objective of the step about to run (diss_site): the step's
+// pre-collision part at every fluid site, nothing stored to f.
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(STEP_TILE_MAX) void k_diss(StepParams p) {
+    step_bulk<P, RLX, ODD, true>(p);
+}
+
+template <int P, int RLX, bool ODD>
+__global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
+    const uint32_t t = blockIdx.x * TILE + threadIdx.x;
+    if (t >= (ODD ? p.nspecial : p.nzh)) return;
+    const uint32_t c = p.special[t];
+    step_site<P, RLX, ODD, MODE_SPECIAL, true>(p, c, p.mask[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -688,8 +770,9 @@ struct AdjParams {
     double* gface;      // accumulated face-velocity cotangent (face of gface_id), face layout x 3
     int gface_id;
     int seed;           // 1 on the objective's step
-    int obj_face, obj_axis;
+    int obj_face, obj_axis;  // obj_face LB_OBJ_DISSIPATION: the dissipation objective
     double obj_scale;   // 1 / (fluid sites on the objective face)
+    const void* dseed;  // dissipation objective: the seeds of diss_site
 };
 
 // Per-site core of the adjoint step: the collision's vector-Jacobian product
@@ -711,7 +794,14 @@ LBD void adj_core(const StepParams& p, const AdjParams& x, const uint32_t c, con
     TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
     TC ga[3] = {TC(0), TC(0), TC(0)};
     collide_vjp<RLX, TC>(lam, rho, us, acc, prm, mu, ga);
-    if (x.seed) {
+    if (x.seed && x.obj_face == LB_OBJ_DISSIPATION) {
+        // dissipation objective: seeds written by diss_site in the forward pass
+        const TC* S = (const TC*)x.dseed;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) mu[l] += S[(int64_t)l * nq + c];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ga[a] += S[(int64_t)(Q + a) * nq + c];
+    } else if (x.seed) {
         // objective: mean over the face layer of rho u*_axis = j_axis + rho a_axis / 2
         const int ax = x.obj_face >> 1;
         const int pos = ax == 0 ? i : ax == 1 ? j : k;
@@ -1289,6 +1379,7 @@ struct lb_solver {
     int adj_face = FX1, adj_axis = 0;
     double adj_scale = 0.0;
     bool adj_ready = false;
+    void* dseed = nullptr;  // dissipation objective: (Q + 4) rows of nq, compute type
     // one snapshot slot (lb_snapshot / lb_restore)
     void* snap = nullptr;
     double* snap_dv = nullptr;
@@ -1723,6 +1814,39 @@ static void launch_adj_rlx(const lb_solver* s, const StepParams& p, const AdjPar
     else launch_adj<P, REGUL>(s, p, x, odd);
 }
 
+// the dissipation objective's gather before the step of parity `odd`
+template <int P, int RLX>
+static void launch_diss(const StepParams& p, bool odd, hipStream_t st) {
+    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    if (odd) k_diss<P, RLX, true><<<grid, p.tile, 0, st>>>(p);
+    else k_diss<P, RLX, false><<<grid, p.tile, 0, st>>>(p);
+    const uint32_t ns = odd ? p.nspecial : p.nzh;
+    if (ns) {
+        const unsigned g2 = (ns + TILE - 1) / TILE;
+        if (odd) k_diss_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
+        else k_diss_special<P, RLX, false><<<g2, TILE, 0, st>>>(p);
+    }
+}
+
+template <int P>
+static void launch_diss_rlx(const lb_solver* s, const StepParams& p, bool odd) {
+    if (s->cfg.rlx_scheme == BGK) launch_diss<P, BGK>(p, odd, s->stream);
+    else if (s->cfg.rlx_scheme == TRT) launch_diss<P, TRT>(p, odd, s->stream);
+    else launch_diss<P, REGUL>(p, odd, s->stream);
+}
+
+// sum of one nq-long row of the compute type, 256 partial sums into part
+template <class TC>
+__global__ __launch_bounds__(256) void k_sum_row(const TC* v, uint32_t n, double* part) {
+    double a = 0.0;
+    for (uint32_t c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) a += (double)v[c];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1877,7 +2001,7 @@ int lb_destroy(lb_solver* s) {
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
                     s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
                     s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab,
-                    s->tabc, s->tabw};
+                    s->tabc, s->tabw, s->dseed};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {
@@ -2308,7 +2432,9 @@ int lb_objective(lb_solver* s, int face, int axis, double* J) {
 }
 
 int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, int axis, double* J) {
-    if (!s || !J || face < 0 || face > 5 || axis < 0 || axis > 2) return fail(LB_EINVAL, "invalid argument");
+    const bool diss = face == LB_OBJ_DISSIPATION;
+    if (!s || !J || face < 0 || face > LB_OBJ_DISSIPATION || (!diss && (axis < 0 || axis > 2)))
+        return fail(LB_EINVAL, "invalid argument");
     if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
     if (steps < 1 || interval < 1) return fail(LB_EINVAL, "steps and checkpoint interval must be >= 1");
     if (s->prec != LB_FP64 && s->prec != LB_FP32_SHIFTED)
@@ -2326,6 +2452,16 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         if ((rc = dalloc(s, &s->ckpt, (size_t)(nseg + 1) * fbytes))) return rc;
         s->ckpt_bytes = (size_t)(nseg + 1) * fbytes;
     }
+    if (diss && s->sparse) return fail(LB_EINVAL, "the dissipation objective runs on the dense layouts");
+    const size_t tcs = tc_size(s->prec);
+    const size_t drows = (size_t)(Q + 4) * s->nq;
+    if (diss) {
+        if (!s->dseed && (rc = dalloc(s, &s->dseed, drows * tcs))) return rc;
+        const unsigned fb = 256;
+        if (s->prec == LB_FP64)
+            k_fill<double><<<(unsigned)((drows + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->dseed, drows, 0.0);
+        else k_fill<float><<<(unsigned)((drows + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->dseed, drows, 0.0f);
+    }
     s->adj_po = s->next_odd;
     StepParams p = params(s);
     for (uint32_t seg = 0; seg < nseg; ++seg) {
@@ -2333,6 +2469,13 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         const uint32_t len = std::min(interval, steps - seg * interval);
         for (uint32_t t = 0; t < len; ++t) {
             p.store_dv = (seg * interval + t + 1 == steps) ? 1 : 0;
+            if (diss && p.store_dv) {  // the objective is on the final step's collision input
+                StepParams q = p;
+                q.dseed = s->dseed;
+                q.dscale = 1.0 / (double)s->nfluid;
+                if (s->prec == LB_FP64) launch_diss_rlx<LB_FP64>(s, q, s->next_odd != 0);
+                else launch_diss_rlx<LB_FP32_SHIFTED>(s, q, s->next_odd != 0);
+            }
             launch_any_step(s, p, s->next_odd != 0);
             s->next_odd ^= 1;
             s->steps++;
@@ -2341,7 +2484,23 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync((char*)s->ckpt + (size_t)nseg * fbytes, s->f, fbytes, hipMemcpyDeviceToDevice, s->stream));
     double n = 0.0;
-    if ((rc = objective(s, face, axis, J, &n))) return rc;
+    if (diss) {
+        const int nb = 256;
+        const size_t off = (size_t)(Q + 3) * s->nq;
+        if (s->prec == LB_FP64)
+            k_sum_row<double><<<nb, 256, 0, s->stream>>>((const double*)s->dseed + off, s->nq, s->part);
+        else k_sum_row<float><<<nb, 256, 0, s->stream>>>((const float*)s->dseed + off, s->nq, s->part);
+        HIPCHK(hipGetLastError());
+        std::vector<double> h(nb);
+        HIPCHK(hipMemcpyAsync(h.data(), s->part, nb * 8, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        double a = 0.0;
+        for (double v : h) a += v;
+        n = (double)s->nfluid;
+        *J = a / n;
+    } else if ((rc = objective(s, face, axis, J, &n))) {
+        return rc;
+    }
     if (n <= 0) return fail(LB_EINVAL, "objective face %d has no fluid site", face);
     s->adj_steps = steps;
     s->adj_interval = interval;
@@ -2400,6 +2559,7 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     x.obj_face = s->adj_face;
     x.obj_axis = s->adj_axis;
     x.obj_scale = s->adj_scale;
+    x.dseed = s->dseed;
     for (int seg = (int)nseg - 1; seg >= 0; --seg) {
         HIPCHK(hipMemcpyAsync(s->f, (char*)s->ckpt + (size_t)seg * fbytes, fbytes, hipMemcpyDeviceToDevice, s->stream));
         const uint32_t g0 = (uint32_t)seg * interval;

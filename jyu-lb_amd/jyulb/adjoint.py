@@ -8,7 +8,13 @@ isothermal.pyx:1293-1295) and the velocity data of a Zou-He face
     J = mean over the fluid sites of a face layer of rho * u*_axis
 
 (e.g. face X1, axis X: the mean outlet x-flux), evaluated on the density and
-half-force shifted velocity of the last step. The reverse pass is the
+half-force shifted velocity of the last step, or the dissipation objective
+
+    J = mean over the fluid sites of Pi_neq : Pi_neq
+
+with Pi_neq = sum_l c_l c_l f_l - rho u* u* - rho/3 I the non-equilibrium
+momentum flux of the last step's collision input (the viscous stress is
+-(1 - omega/2) Pi_neq). The reverse pass is the
 hand-written adjoint AA step of libjyulb_hip.so with checkpoints every
 `interval` steps and recomputation (include/jyulb_gpu.h: lb_adj_forward,
 lb_adj_backward). Parameters are held fixed over the steps; the populations at
@@ -20,6 +26,7 @@ import numpy as np
 from .defs import FACE_X1, X, FIXED_DEN, FIXED_VEL
 
 _NAMES = ('1X', 'X1', '1Y', 'Y1', '1Z', 'Z1')
+OBJ_DISSIPATION = 6  # LB_OBJ_DISSIPATION of include/jyulb_gpu.h
 
 
 def _prepare(solver):
@@ -45,6 +52,16 @@ def flux_gradient(solver, steps, interval=50, face=FACE_X1, axis=X, grad_face=No
     _prepare(solver)
     eng = solver.engine
     J = eng.adj_forward(steps, interval, face, axis)
+    ga, gf = eng.adj_backward(-1 if grad_face is None else int(grad_face))
+    _finish(solver, steps)
+    return J, ga, gf
+
+
+def dissipation_gradient(solver, steps, interval=50, grad_face=None):
+    """Like flux_gradient, for the dissipation objective (dense layouts)."""
+    _prepare(solver)
+    eng = solver.engine
+    J = eng.adj_forward(steps, interval, OBJ_DISSIPATION, 0)
     ga, gf = eng.adj_backward(-1 if grad_face is None else int(grad_face))
     _finish(solver, steps)
     return J, ga, gf

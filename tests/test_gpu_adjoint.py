@@ -48,32 +48,37 @@ def _inputs(s):
             s.bc_data_vel_face_1X.copy(), np.array(s.gravity))
 
 
-def _J(eng, acc, fden, fvel, g, steps):
+OBJ_DISS = 6  # LB_OBJ_DISSIPATION
+
+
+def _J(eng, acc, fden, fvel, g, steps, obj=FACE_X1):
     eng.restore()
     eng.set_gravity(g)
     eng.upload_acc(acc)
     eng.set_face_data(FACE_1X, fden, fvel)
+    if obj == OBJ_DISS:  # evaluated on the final step's collision input
+        return eng.adj_forward(steps, steps, OBJ_DISS, 0)
     eng.evolve(steps)
     return eng.objective(FACE_X1, X)
 
 
-def _adjoint(s, acc, fden, fvel, g, steps, interval):
+def _adjoint(s, acc, fden, fvel, g, steps, interval, obj=FACE_X1):
     eng = s.engine
     eng.restore()
     eng.set_gravity(g)
     eng.upload_acc(acc)
     eng.set_face_data(FACE_1X, fden, fvel)
-    J = eng.adj_forward(steps, interval, FACE_X1, X)
+    J = eng.adj_forward(steps, interval, obj, X)
     ga, gf = eng.adj_backward(FACE_1X)
     return J, ga, gf
 
 
-def _fd_checks(s_adj, s_fd, prof, amp, steps, interval, ncells, seed):
+def _fd_checks(s_adj, s_fd, prof, amp, steps, interval, ncells, seed, obj=FACE_X1):
     acc, fden, fvel, g = _inputs(s_fd)
     s_fd.engine.snapshot()
     s_adj.engine.snapshot()
-    J, ga, gf = _adjoint(s_adj, acc, fden, fvel, g, steps, interval)
-    J0 = _J(s_fd.engine, acc, fden, fvel, g, steps)
+    J, ga, gf = _adjoint(s_adj, acc, fden, fvel, g, steps, interval, obj)
+    J0 = _J(s_fd.engine, acc, fden, fvel, g, steps, obj)
     h = 1e-6
     rng = np.random.default_rng(seed)
     fl = np.argwhere(s_fd.get_phase() != 0)
@@ -85,14 +90,15 @@ def _fd_checks(s_adj, s_fd, prof, amp, steps, interval, ncells, seed):
         ap[i, j, k, a] += h
         am = acc.copy()
         am[i, j, k, a] -= h
-        fd.append((_J(s_fd.engine, ap, fden, fvel, g, steps) - _J(s_fd.engine, am, fden, fvel, g, steps)) / (2 * h))
+        fd.append((_J(s_fd.engine, ap, fden, fvel, g, steps, obj) - _J(s_fd.engine, am, fden, fvel, g, steps, obj))
+                  / (2 * h))
         adj.append(ga[i, j, k, a])
     # inlet amplitude: v = A * profile (x component)
     vp = fvel.copy()
     vp[:, :, 0] = (amp + h) * prof
     vm = fvel.copy()
     vm[:, :, 0] = (amp - h) * prof
-    fdA = (_J(s_fd.engine, acc, fden, vp, g, steps) - _J(s_fd.engine, acc, fden, vm, g, steps)) / (2 * h)
+    fdA = (_J(s_fd.engine, acc, fden, vp, g, steps, obj) - _J(s_fd.engine, acc, fden, vm, g, steps, obj)) / (2 * h)
     adjA = float(np.sum(gf[:, :, 0] * prof))
     return J, J0, np.array(adj), np.array(fd), adjA, fdA
 
@@ -289,3 +295,40 @@ def test_adjoint_porous_channel_sparse_equals_rows():
     assert out[0][0] == out[1][0]
     np.testing.assert_array_equal(out[1][1], out[0][1])
     np.testing.assert_array_equal(out[1][2], out[0][2])
+
+
+@pytest.mark.parametrize("rlx", [BGK, TRT, REGULARIZATION], ids=["bgk", "trt", "reg"])
+def test_dissipation_adjoint_fp64_matches_finite_differences(rlx):
+    """The dissipation objective (mean Pi_neq:Pi_neq of the final step's
+    collision input, seeded by k_diss) against FP64 central differences, for
+    the acceleration field and the Zou-He inlet amplitude."""
+    print(f"dissipation adjoint fp64 rlx={rlx}", flush=True)
+    s, prof, amp = make_channel(rlx, "fp64")
+    J, J0, adj, fd, adjA, fdA = _fd_checks(s, s, prof, amp, steps=60, interval=16, ncells=16, seed=9, obj=OBJ_DISS)
+    assert J > 0 and abs(J - J0) <= 1e-13 * abs(J0)
+    scale = np.max(np.abs(fd))
+    assert scale > 0
+    err = np.max(np.abs(adj - fd)) / scale
+    assert err < 1e-6, (err, adj, fd)
+    assert abs(adjA - fdA) <= 1e-6 * abs(fdA), (adjA, fdA)
+
+
+def test_dissipation_adjoint_fp32_within_1e4_of_fp64_fd():
+    s32, prof, amp = make_channel(TRT, "fp32")
+    s64, _, _ = make_channel(TRT, "fp64")
+    J, J0, adj, fd, adjA, fdA = _fd_checks(s32, s64, prof, amp, steps=60, interval=16, ncells=16, seed=13,
+                                           obj=OBJ_DISS)
+    scale = np.max(np.abs(fd))
+    err = np.max(np.abs(adj - fd)) / scale
+    assert err < 1e-4, (err, adj, fd)
+    assert abs(adjA - fdA) <= 1e-4 * abs(fdA), (adjA, fdA)
+
+
+def test_dissipation_gradient_python_helper_and_sparse_refusal():
+    from jyulb import adjoint
+    s, prof, amp = make_channel(TRT, "fp64")
+    J, ga, gf = adjoint.dissipation_gradient(s, 20, 8, grad_face=FACE_1X)
+    assert J > 0 and np.isfinite(ga).all() and np.isfinite(gf).all()
+    sp, _, _ = make_channel(TRT, "fp64", layout="sparse")
+    with pytest.raises(ValueError, match="dense layouts"):
+        adjoint.dissipation_gradient(sp, 20, 8)

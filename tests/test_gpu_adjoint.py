@@ -332,3 +332,31 @@ def test_dissipation_gradient_python_helper_and_sparse_refusal():
     sp, _, _ = make_channel(TRT, "fp64", layout="sparse")
     with pytest.raises(ValueError, match="dense layouts"):
         adjoint.dissipation_gradient(sp, 20, 8)
+
+
+@pytest.mark.parametrize("rlx", [BGK, TRT, REGULARIZATION], ids=["bgk", "trt", "reg"])
+def test_dissipation_adjoint_matches_torch_fp64_autograd(rlx):
+    """The dissipation objective's gradients against torch.autograd through
+    the FP64 PyTorch restatement (tests/torch_lb.py)."""
+    import torch
+    import torch_lb
+    lat = (10, 6, 5)
+    s, prof, amp = make_channel(rlx, "fp64", lat=lat, steps_bc=(WALL, SYMMETRIC), seed=23)
+    acc, fden, fvel, g = _inputs(s)
+    s.engine.snapshot()
+    f0 = s.post_collision_f()[:, 1:-1, 1:-1, 1:-1].reshape(19, -1)
+    steps = 30
+    J, ga, gf = _adjoint(s, acc, fden, fvel, g, steps, 7, obj=OBJ_DISS)
+    ref = torch_lb.TorchLB(np.asarray(s.get_phase()).copy(), s.bc, rlx, 0.2 / 3.0)
+    acc_t = torch.tensor(acc.reshape(-1, 3), dtype=torch.float64, requires_grad=True)
+    v_t = torch.tensor(fvel.reshape(-1, 3), dtype=torch.float64, requires_grad=True)
+    ref.run(torch.tensor(f0), acc_t, v_t, 1.0, torch.tensor(g), steps)
+    Jt = ref.dissipation()
+    Jt.backward()
+    assert abs(float(Jt.detach()) - J) <= 1e-11 * abs(J), (float(Jt.detach()), J)
+    gt = acc_t.grad.numpy().reshape(ga.shape)
+    fluid = np.asarray(s.get_phase()) != 0
+    scale = np.max(np.abs(gt[fluid]))
+    assert np.max(np.abs(ga[fluid] - gt[fluid])) <= 1e-9 * scale
+    gvt = v_t.grad.numpy().reshape(gf.shape)
+    np.testing.assert_allclose(gf[1:-1, 1:-1], gvt[1:-1, 1:-1], rtol=0, atol=1e-9 * np.max(np.abs(gvt)))

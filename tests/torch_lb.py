@@ -166,13 +166,29 @@ class TorchLB:
     def run(self, f0, acc, v_in, den_out, g, steps):
         """f0: (19, nc) post-collision state; acc: (nc, 3) field; v_in:
         ((ny+2)*(nz+2), 3) face-1X velocity data. Returns (rho, u*) of the
-        last step at the fluid sites (in the order of self.fluid)."""
+        last step at the fluid sites (in the order of self.fluid); the last
+        step's collision input and acceleration stay in self.last_input."""
         f = f0
         a = acc[self.fluid] + g.unsqueeze(0)
         rho = us = None
         for _ in range(steps):
             gsite = f.reshape(-1)[self.src]
             gsite = self._zouhe(gsite, v_in, den_out)
+            self.last_input = (gsite, a)
             fpost, rho, us = self._collide(gsite, a)
             f = f.index_put((self.rows, self.fluid.unsqueeze(0)), fpost)
         return rho, us
+
+    def dissipation(self):
+        """Mean over the fluid sites of Pi_neq : Pi_neq of the last step's
+        collision input, Pi_neq = sum_l c_l c_l g_l - rho u* u* - rho/3 I."""
+        g, a = self.last_input
+        c = torch.tensor([LI, LJ, LK], dtype=g.dtype, device=g.device)
+        rho = g.sum(0)
+        us = (c @ g) / rho + 0.5 * a.T
+        tot = 0
+        for p in range(3):
+            for q in range(3):
+                pn = (c[p] * c[q]) @ g - rho * us[p] * us[q] - (rho / 3 if p == q else 0)
+                tot = tot + pn * pn
+        return tot.mean()

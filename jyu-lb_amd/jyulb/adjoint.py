@@ -140,3 +140,10 @@ def flux_objective(acc, face_vel, solver, steps, interval=50, face=FACE_X1, axis
         inlet_face = fixed_face(solver)
     return _FN.apply(acc, face_vel, solver, int(steps), int(interval), int(face), int(axis),
                      -1 if inlet_face is None else int(inlet_face))
+
+
+def dissipation_objective(acc, face_vel, solver, steps, interval=50, inlet_face=None):
+    """torch entry point for the dissipation objective (flux_objective with
+    face = OBJ_DISSIPATION): mean Pi_neq : Pi_neq of the last step as a
+    differentiable function of `acc` and (optionally) `face_vel`."""
+    return flux_objective(acc, face_vel, solver, steps, interval, OBJ_DISSIPATION, 0, inlet_face)

@@ -360,3 +360,22 @@ def test_dissipation_adjoint_matches_torch_fp64_autograd(rlx):
     assert np.max(np.abs(ga[fluid] - gt[fluid])) <= 1e-9 * scale
     gvt = v_t.grad.numpy().reshape(gf.shape)
     np.testing.assert_allclose(gf[1:-1, 1:-1], gvt[1:-1, 1:-1], rtol=0, atol=1e-9 * np.max(np.abs(gvt)))
+
+
+def test_dissipation_objective_torch_function():
+    """The autograd Function form of the dissipation objective returns the
+    gradients of dissipation_gradient."""
+    import torch
+    from jyulb import adjoint
+    s, prof, amp = make_channel(TRT, "fp64", seed=17)
+    acc, fden, fvel, g = _inputs(s)
+    s.engine.snapshot()
+    J0, ga0, gf0 = _adjoint(s, acc, fden, fvel, g, 24, 8, obj=OBJ_DISS)
+    s.engine.restore()
+    a = torch.tensor(acc, device="cuda", requires_grad=True)
+    v = torch.tensor(fvel, device="cuda", requires_grad=True)
+    J = adjoint.dissipation_objective(a, v, s, 24, 8, inlet_face=FACE_1X)
+    J.backward()
+    assert abs(float(J.detach()) - J0) <= 1e-14 * abs(J0)
+    np.testing.assert_allclose(a.grad.cpu().numpy(), ga0, rtol=0, atol=1e-14 * np.max(np.abs(ga0)))
+    np.testing.assert_allclose(v.grad.cpu().numpy(), gf0, rtol=0, atol=1e-14 * np.max(np.abs(gf0)))

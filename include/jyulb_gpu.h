@@ -147,7 +147,7 @@ int lb_objective(lb_solver *s, int face, int axis, double *J);
 /* Objective selector of lb_adj_forward: instead of a face flux, the mean over
  * the fluid sites of Pi_neq:Pi_neq, the squared non-equilibrium momentum flux
  * (proportional to the viscous dissipation) of the last step's collision
- * input; `axis` is ignored. Dense layouts. */
+ * input; `axis` is ignored. */
 #define LB_OBJ_DISSIPATION 6
 
 /* Adjoint, forward pass (no counterpart in the reference: reverse-mode

@@ -680,7 +680,7 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
     }
 }
 
-template <int P, int RLX, bool ODD, bool FULL>
+template <int P, int RLX, bool ODD, bool FULL, bool DISS = false>
 LBD void step_sparse(const StepParams& p, const uint32_t n) {
     using TS = typename Var<P>::TS;
     using TC = typename Var<P>::TC;
@@ -688,7 +688,7 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
     TS* F = (TS*)p.f;
     const uint32_t ps = (uint32_t)p.ps;  // 19 * ps < 2^32 (lb_finalize)
     const TC* A = (const TC*)p.acc;
-    const bool need_c = FULL || A || p.store_dv || p.rec;  // uniform
+    const bool need_c = FULL || DISS || A || p.store_dv || p.rec;  // uniform
     const uint32_t c = need_c ? p.site[n] : 0u;
     const uint32_t m = FULL ? p.mask[c] : M_FLUID;
     constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
@@ -718,6 +718,10 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
     TC acc[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
+    if constexpr (DISS) {  // objective gather: nothing is stored to f
+        diss_site<TC, SH>(p, c, f, acc);
+        return;
+    }
     TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
     TC vel[3];
     const TC den = collide<RLX, TC, SH>(f, acc, prm, rho0, vel);
@@ -752,6 +756,16 @@ __global__ __launch_bounds__(STEP_TILE_MAX) void k_step_sp(StepParams p) {
     const uint32_t n = tile * p.tile + threadIdx.x;
     if (n >= p.nf) return;
     step_sparse<P, RLX, ODD, FULL>(p, n);
+}
+
+// the dissipation objective's gather in the sparse layout (see k_diss)
+template <int P, int RLX, bool ODD, bool FULL>
+__global__ __launch_bounds__(STEP_TILE_MAX) void k_diss_sp(StepParams p) {
+    const uint32_t tile = tile_index(p);
+    if (tile >= p.ntiles) return;
+    const uint32_t n = tile * p.tile + threadIdx.x;
+    if (n >= p.nf) return;
+    step_sparse<P, RLX, ODD, FULL, true>(p, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -1818,6 +1832,16 @@ static void launch_adj_rlx(const lb_solver* s, const StepParams& p, const AdjPar
 template <int P, int RLX>
 static void launch_diss(const StepParams& p, bool odd, hipStream_t st) {
     const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    if (p.tab) {  // sparse layout
+        if (p.moving || p.nzh) {
+            if (odd) k_diss_sp<P, RLX, true, true><<<grid, p.tile, 0, st>>>(p);
+            else k_diss_sp<P, RLX, false, true><<<grid, p.tile, 0, st>>>(p);
+        } else {
+            if (odd) k_diss_sp<P, RLX, true, false><<<grid, p.tile, 0, st>>>(p);
+            else k_diss_sp<P, RLX, false, false><<<grid, p.tile, 0, st>>>(p);
+        }
+        return;
+    }
     if (odd) k_diss<P, RLX, true><<<grid, p.tile, 0, st>>>(p);
     else k_diss<P, RLX, false><<<grid, p.tile, 0, st>>>(p);
     const uint32_t ns = odd ? p.nspecial : p.nzh;
@@ -2452,7 +2476,6 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         if ((rc = dalloc(s, &s->ckpt, (size_t)(nseg + 1) * fbytes))) return rc;
         s->ckpt_bytes = (size_t)(nseg + 1) * fbytes;
     }
-    if (diss && s->sparse) return fail(LB_EINVAL, "the dissipation objective runs on the dense layouts");
     const size_t tcs = tc_size(s->prec);
     const size_t drows = (size_t)(Q + 4) * s->nq;
     if (diss) {

@@ -58,7 +58,7 @@ def flux_gradient(solver, steps, interval=50, face=FACE_X1, axis=X, grad_face=No
 
 
 def dissipation_gradient(solver, steps, interval=50, grad_face=None):
-    """Like flux_gradient, for the dissipation objective (dense layouts)."""
+    """Like flux_gradient, for the dissipation objective."""
     _prepare(solver)
     eng = solver.engine
     J = eng.adj_forward(steps, interval, OBJ_DISSIPATION, 0)

@@ -324,14 +324,18 @@ def test_dissipation_adjoint_fp32_within_1e4_of_fp64_fd():
     assert abs(adjA - fdA) <= 1e-4 * abs(fdA), (adjA, fdA)
 
 
-def test_dissipation_gradient_python_helper_and_sparse_refusal():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dissipation_gradient_sparse_equals_rows(precision):
+    """jyulb.adjoint.dissipation_gradient in both layouts: objective and
+    gradients bit-identical."""
     from jyulb import adjoint
-    s, prof, amp = make_channel(TRT, "fp64")
-    J, ga, gf = adjoint.dissipation_gradient(s, 20, 8, grad_face=FACE_1X)
-    assert J > 0 and np.isfinite(ga).all() and np.isfinite(gf).all()
-    sp, _, _ = make_channel(TRT, "fp64", layout="sparse")
-    with pytest.raises(ValueError, match="dense layouts"):
-        adjoint.dissipation_gradient(sp, 20, 8)
+    out = []
+    for layout in ("rows", "sparse"):
+        s, prof, amp = make_channel(TRT, precision, seed=19, layout=layout)
+        out.append(adjoint.dissipation_gradient(s, 20, 8, grad_face=FACE_1X))
+    assert out[0][0] > 0 and out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][2], out[0][2])
 
 
 @pytest.mark.parametrize("rlx", [BGK, TRT, REGULARIZATION], ids=["bgk", "trt", "reg"])

@@ -169,7 +169,7 @@ def sparse_workload(steps, precs, lat=(384, 192, 192)):
         row = {}
         for layout in ("rows", "sparse"):
             s, nfl = porous_medium(prec, lat=lat, layout=layout, solid=solid)
-            t, _ = time_device(s, steps, 10)
+            t, _ = time_device(s, steps, 40)  # the warmup holds the launch tuning
             row[layout] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
                                ms_per_step=round(1e3 * t / steps, 4))
             del s
@@ -194,7 +194,7 @@ def throughput_workload(steps, precs, layouts=("rows",)):
         s, nfl = porous_medium(prec, layout=layout)
         key = prec if layout == "rows" else f"{prec}:{layout}"
         print(f"[bench] porous 768x384x384 {key}: {nfl} fluid sites, {steps} steps", file=sys.stderr, flush=True)
-        t, _ = time_device(s, steps, 10)
+        t, _ = time_device(s, steps, 40)  # the warmup holds the launch tuning
         out[key] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
                          ms_per_step=round(1e3 * t / steps, 4),
                          achieved_gbs=round(BYTES_PER_SITE[prec] * nfl * steps / t / 1e9, 1),

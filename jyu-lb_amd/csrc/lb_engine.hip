@@ -1342,6 +1342,7 @@ struct lb_solver {
     uint32_t step_lds[2] = {0, 0};            // see StepParams::lds
     uint32_t step_tile = TILE;                // see StepParams::tile
     bool tune = false;                        // choose the step launch on the next long evolve (tune_step)
+    bool tune_xcd = true;                     // ... including the tile order (not pinned by the user)
     bool tune_verbose = false;
     uint32_t* mov_idx = nullptr;  // moving-site index per site (k_ladd_prep)
     double* ladd_u = nullptr;     // 18 Ladd velocity sums per moving site
@@ -1447,7 +1448,7 @@ static StepParams params(const lb_solver* s) {
     p.nq = s->nq;
     p.tile = s->step_tile;
     p.ntiles = ((s->sparse ? s->nfp : s->nq) + p.tile - 1) / p.tile;
-    p.xcd = s->nq >= s->xcd_cells ? 1u : 0u;
+    p.xcd = s->nq >= s->xcd_cells ? 1u : 0u;  // the tuned launch may set xcd_cells to 0 or UINT64_MAX
     p.lds[0] = s->step_lds[0];
     p.lds[1] = s->step_lds[1];
     p.rs = s->rs;
@@ -1908,7 +1909,10 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     // with the XCD order, FP16 the same either way).
     if (cfg->layout == LB_LAYOUT_SPARSE && (s->prec == LB_FP32_SHIFTED || s->prec == LB_FP32_STORAGE))
         s->xcd_cells = UINT64_MAX;
-    if (const char* env = getenv("LB_XCD_REMAP_CELLS")) s->xcd_cells = strtoull(env, nullptr, 10);
+    if (const char* env = getenv("LB_XCD_REMAP_CELLS")) {
+        s->xcd_cells = strtoull(env, nullptr, 10);
+        s->tune_xcd = false;
+    }
     // Unused dynamic LDS caps the bulk step's occupancy. The FP32 kernel needs
     // few registers (8 waves per SIMD); its even step streamed 3-4% faster at
     // 6 waves per SIMD (26 KB per workgroup of 256 threads), while capping the
@@ -2258,32 +2262,45 @@ static void run_steps(lb_solver* s, uint32_t n, bool store_last) {
 // are timed on the call's own steps.
 struct StepCfg {
     uint32_t tile, lds_even, lds_odd;
+    uint64_t xcd_cells;  // see tile_index: 0 = XCD order, UINT64_MAX = round-robin
 };
 constexpr int TUNE_WARM = 2, TUNE_STEPS = 4;
 
+// the XCD tile order is a candidate too on lattices of at least 2^24 cells
+constexpr uint64_t TUNE_XCD_CELLS = 1ull << 24;
+constexpr int TUNE_MAX = 10;
+
 static int tune_candidates(const lb_solver* s, StepCfg* c) {
     int n = 0;
-    c[n++] = {s->step_tile, s->step_lds[0], s->step_lds[1]};  // the default
+    const uint64_t xd = s->nq >= s->xcd_cells ? 0 : UINT64_MAX;  // the default order
+    c[n++] = {s->step_tile, s->step_lds[0], s->step_lds[1], xd};
     auto add = [&](StepCfg x) {
         for (int i = 0; i < n; ++i)
-            if (c[i].tile == x.tile && c[i].lds_even == x.lds_even && c[i].lds_odd == x.lds_odd) return;
-        c[n++] = x;
+            if (c[i].tile == x.tile && c[i].lds_even == x.lds_even && c[i].lds_odd == x.lds_odd &&
+                c[i].xcd_cells == x.xcd_cells)
+                return;
+        if (n < TUNE_MAX) c[n++] = x;
     };
-    if (s->sparse) return n;
-    if (s->prec == LB_FP64 || s->prec == LB_FP32_STORAGE) {
-        add({512, 0, 0});
-        add({256, 0, 0});
-        add({256, 40000, 40000});  // 4 workgroups per CU: 4 waves per SIMD
-        add({256, 50000, 50000});  // 3 waves per SIMD
-    } else if (s->prec == LB_FP32_SHIFTED) {
-        add({256, 0, 0});
-        add({256, 26000, 0});
+    if (!s->sparse) {
+        if (s->prec == LB_FP64 || s->prec == LB_FP32_STORAGE) {
+            add({512, 0, 0, xd});
+            add({256, 0, 0, xd});
+            add({256, 40000, 40000, xd});  // 4 workgroups per CU: 4 waves per SIMD
+            add({256, 50000, 50000, xd});  // 3 waves per SIMD
+        } else if (s->prec == LB_FP32_SHIFTED) {
+            add({256, 0, 0, xd});
+            add({256, 26000, 0, xd});
+        }
+    }
+    if (s->tune_xcd && (uint64_t)s->nq >= TUNE_XCD_CELLS) {  // each of them in the other tile order
+        const int m = n;
+        for (int i = 0; i < m; ++i) add({c[i].tile, c[i].lds_even, c[i].lds_odd, xd ? 0 : UINT64_MAX});
     }
     return n;
 }
 
 static int tune_step(lb_solver* s) {
-    StepCfg c[6];
+    StepCfg c[TUNE_MAX];
     const int n = tune_candidates(s, c);
     s->tune = false;
     if (n < 2) return LB_OK;
@@ -2301,6 +2318,7 @@ static int tune_step(lb_solver* s) {
         s->step_tile = c[i].tile;
         s->step_lds[0] = c[i].lds_even;
         s->step_lds[1] = c[i].lds_odd;
+        s->xcd_cells = c[i].xcd_cells;
         HIPCHK(hipEventRecord(ev[2 * i], s->stream));
         run_steps(s, TUNE_STEPS, false);
         HIPCHK(hipEventRecord(ev[2 * i + 1], s->stream));
@@ -2317,13 +2335,14 @@ static int tune_step(lb_solver* s) {
             tbest = ms;
         }
         if (s->tune_verbose)
-            fprintf(stderr, "[jyulb] step launch tile %u, LDS %u/%u: %.4f ms per step\n", c[i].tile, c[i].lds_even,
-                    c[i].lds_odd, ms / TUNE_STEPS);
+            fprintf(stderr, "[jyulb] step launch tile %u, LDS %u/%u, %s order: %.4f ms per step\n", c[i].tile,
+                    c[i].lds_even, c[i].lds_odd, c[i].xcd_cells ? "round-robin" : "XCD", ms / TUNE_STEPS);
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     s->step_tile = c[best].tile;
     s->step_lds[0] = c[best].lds_even;
     s->step_lds[1] = c[best].lds_odd;
+    s->xcd_cells = c[best].xcd_cells;
     return LB_OK;
 }
 
@@ -2334,7 +2353,7 @@ int lb_evolve(lb_solver* s, uint32_t steps) {
     if (s->tune) {
         // tuning takes TUNE_WARM + TUNE_STEPS per candidate of this call's
         // steps; a shorter call leaves it to a later one
-        StepCfg c[6];
+        StepCfg c[TUNE_MAX];
         const int n = tune_candidates(s, c);
         if (n < 2) s->tune = false;
         else if (steps > (uint32_t)(TUNE_WARM + n * TUNE_STEPS)) {

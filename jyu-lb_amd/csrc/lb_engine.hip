@@ -1478,6 +1478,11 @@ static unsigned blocks_for(uint32_t n, unsigned b) { return (n + b - 1) / b; }
 // _check_bc, isothermal.pyx:378-426
 static int check_config(const lb_config* c) {
     if (c->lat[0] < 1 || c->lat[1] < 1 || c->lat[2] < 1) return fail(LB_EINVAL, "Invalid lattice size input.");
+    // sites are indexed with 32 bits: nx * ny * round_up(nz, 64) < 2^32
+    const uint64_t padded = (uint64_t)c->lat[0] * (uint64_t)c->lat[1] * (((uint64_t)c->lat[2] + 63) / 64 * 64);
+    if (padded >= (1ull << 32))
+        return fail(LB_EINVAL, "lattice of %llu cells (z padded to a multiple of 64) exceeds the engine's 2^32 limit",
+                    (unsigned long long)padded);
     if (c->rlx_scheme < 0 || c->rlx_scheme > 2) return fail(LB_EINVAL, "Invalid relaxation scheme input.");
     if (!(c->kvisc > 0.0)) return fail(LB_EINVAL, "Invalid kinematic viscosity input.");
     for (int f = 0; f < 6; ++f)

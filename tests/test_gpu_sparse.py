@@ -150,3 +150,15 @@ def test_sparse_device_memory_follows_fluid_sites():
     # u32 bases, and a compact index per cell
     sparse = (19 * 8 + 4 + 18 * 4 + 18) * nfp + 18 * 4 * (nfp // 64) + 4 * ncell
     assert b_rows - b_sp == 19 * 8 * ncell - sparse
+
+
+@pytest.mark.parametrize("layout", ["rows", "sparse"])
+def test_lattice_beyond_32_bit_site_index_is_refused(layout):
+    """Sites are indexed with 32 bits; a lattice of 2^32 or more padded cells
+    is refused by lb_create before it allocates anything (the engine handle
+    is created directly: the Solver would first build its host arrays)."""
+    from jyulb import _native
+    from jyulb.defs import BGK, PERIODIC
+    with pytest.raises(ValueError, match="2\\^32"):
+        _native.Engine((4096, 1024, 1024), BGK, 0.1, np.zeros(3), [PERIODIC] * 6, precision="fp16",
+                       layout=layout)

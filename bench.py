@@ -242,7 +242,9 @@ def throughput_workload(steps, precs, layouts=("rows",)):
 def time_adjoint(precision, steps, interval):
     """Forward evolve vs the adjoint (forward pass with checkpoints, then the
     reverse pass with per-segment recomputation) over the same steps, all on
-    torch's current stream, timed with HIP events."""
+    torch's current stream, timed with HIP events. The forward pass records
+    (rho, u*) over its last segment, so with interval >= steps the reverse
+    pass recomputes nothing."""
     import torch
     from jyulb.defs import FACE_1X, FACE_X1, X
     from jyulb.adjoint import _prepare
@@ -278,6 +280,7 @@ def time_adjoint(precision, steps, interval):
     return dict(workload="porous channel 128x64x64 (spheres r 4..8, 10% solid, 8-layer open buffers at inlet/outlet), BGK tau=0.7, gx=1e-5, "
                          "Zou-He inlet (peak 0.02) / outlet; J = mean outlet x-flux",
                 precision=precision, steps=steps, checkpoint_interval=interval, fluid_sites=nfl, J=J,
+                device_bytes=eng.info()["device_bytes"],  # lattice, checkpoints and (rho, u*) record
                 forward_ms_per_step=round(1e3 * t_fwd / steps, 5),
                 adjoint_forward_ms_per_step=round(1e3 * t_af / steps, 5),
                 adjoint_reverse_ms_per_step=round(1e3 * t_ab / steps, 5),
@@ -402,6 +405,14 @@ def main():
         adj = time_adjoint("fp32", args.adj_steps, args.adj_interval)
         print(f"[bench] adjoint/forward {adj['adjoint_over_forward']} (reverse pass), "
               f"{adj['gradient_over_forward']} (whole gradient)", file=sys.stderr, flush=True)
+        if args.adj_interval < args.adj_steps:
+            # one segment: the forward pass records every step and nothing is recomputed
+            one = time_adjoint("fp32", args.adj_steps, args.adj_steps)
+            adj["one_segment"] = {k: one[k] for k in ("checkpoint_interval", "device_bytes", "forward_ms_per_step",
+                                                      "adjoint_forward_ms_per_step", "adjoint_reverse_ms_per_step",
+                                                      "adjoint_over_forward", "gradient_over_forward")}
+            print(f"[bench] one segment: adjoint/forward {one['adjoint_over_forward']} (reverse pass), "
+                  f"{one['gradient_over_forward']} (whole gradient)", file=sys.stderr, flush=True)
     main_p = "fp64" if "fp64" in res else precs[0]
     m = res[main_p]
     cpu = None

@@ -1394,6 +1394,7 @@ struct lb_solver {
     int adj_face = FX1, adj_axis = 0;
     double adj_scale = 0.0;
     bool adj_ready = false;
+    bool rec_fwd = false;  // rec holds the last segment, recorded by lb_adj_forward
     void* dseed = nullptr;  // dissipation objective: (Q + 4) rows of nq, compute type
     // one snapshot slot (lb_snapshot / lb_restore)
     void* snap = nullptr;
@@ -2053,12 +2054,14 @@ int lb_set_stream(lb_solver* s, void* st) {
 
 int lb_set_gravity(lb_solver* s, const double g[3]) {
     if (!s || !g) return fail(LB_EINVAL, "null argument");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     for (int a = 0; a < 3; ++a) s->cfg.g[a] = g[a];
     return LB_OK;
 }
 
 int lb_upload_phase(lb_solver* s, const uint8_t* phase) {
     if (!s || !phase) return fail(LB_EINVAL, "null argument");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     int rc = ensure_stage(s);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s->stage, phase, (size_t)s->ncell, hipMemcpyHostToDevice, s->stream));
@@ -2071,6 +2074,7 @@ int lb_upload_phase(lb_solver* s, const uint8_t* phase) {
 
 int lb_upload_den_vel(lb_solver* s, const double* den, const double* vel) {
     if (!s || !den || !vel) return fail(LB_EINVAL, "null argument");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     int rc = ensure_stage(s);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s->stage, den, (size_t)s->ncell * 8, hipMemcpyHostToDevice, s->stream));
@@ -2083,6 +2087,7 @@ int lb_upload_den_vel(lb_solver* s, const double* den, const double* vel) {
 
 int lb_upload_acc(lb_solver* s, const double* acc) {
     if (!s) return fail(LB_EINVAL, "null handle");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     if (!acc) {
         if (s->acc) {
             HIPCHK(hipStreamSynchronize(s->stream));
@@ -2102,6 +2107,7 @@ int lb_upload_acc(lb_solver* s, const double* acc) {
 
 int lb_upload_acc_device(lb_solver* s, const double* acc_dev) {
     if (!s || !acc_dev) return fail(LB_EINVAL, "null argument");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     int rc = ensure_stage(s);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s->stage, acc_dev, (size_t)s->ncell * 24, hipMemcpyDeviceToDevice, s->stream));
@@ -2110,6 +2116,7 @@ int lb_upload_acc_device(lb_solver* s, const double* acc_dev) {
 
 int lb_set_face_data(lb_solver* s, int face, const double* den_face, const double* vel_face) {
     if (!s || face < 0 || face > 5 || !den_face || !vel_face) return fail(LB_EINVAL, "invalid argument");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     if (!s->fden[face]) return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", face);
     const size_t n = (size_t)(face < 2 ? s->NJ * s->NK : face < 4 ? s->NI * s->NK : s->NI * s->NJ);
     HIPCHK(hipMemcpyAsync(s->fden[face], den_face, n * 8, hipMemcpyHostToDevice, s->stream));
@@ -2197,6 +2204,7 @@ static int build_sparse(lb_solver* s) {
 
 int lb_finalize(lb_solver* s, int64_t* nfluid) {
     if (!s) return fail(LB_EINVAL, "null handle");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
     s->finalized = false;  // until the populations below are initialised
     int rc = build_mask(s, true);
@@ -2238,6 +2246,7 @@ int lb_finalize(lb_solver* s, int64_t* nfluid) {
 
 int lb_refresh_walls(lb_solver* s) {
     if (!s) return fail(LB_EINVAL, "null handle");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
     int rc = build_mask(s, true);
     if (rc) return rc;
@@ -2479,6 +2488,23 @@ int lb_objective(lb_solver* s, int face, int axis, double* J) {
     return objective(s, face, axis, J, nullptr);
 }
 
+// The (rho, u*) record of one checkpoint segment: `len` steps of 4 * nq values.
+static int ensure_rec(lb_solver* s, uint32_t len) {
+    const size_t recb = (size_t)len * 4 * s->nq * tc_size(s->prec);
+    if (s->rec_bytes >= recb) return LB_OK;
+    s->rec_fwd = false;
+    if (s->rec) {
+        HIPCHK(hipFree(s->rec));
+        s->bytes -= s->rec_bytes;
+        s->rec = nullptr;
+        s->rec_bytes = 0;
+    }
+    int rc = dalloc(s, &s->rec, recb);
+    if (rc) return rc;
+    s->rec_bytes = recb;
+    return LB_OK;
+}
+
 int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, int axis, double* J) {
     const bool diss = face == LB_OBJ_DISSIPATION;
     if (!s || !J || face < 0 || face > LB_OBJ_DISSIPATION || (!diss && (axis < 0 || axis > 2)))
@@ -2501,6 +2527,8 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         s->ckpt_bytes = (size_t)(nseg + 1) * fbytes;
     }
     const size_t tcs = tc_size(s->prec);
+    s->rec_fwd = false;
+    if ((rc = ensure_rec(s, std::min(interval, steps)))) return rc;
     const size_t drows = (size_t)(Q + 4) * s->nq;
     if (diss) {
         if (!s->dseed && (rc = dalloc(s, &s->dseed, drows * tcs))) return rc;
@@ -2516,8 +2544,11 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         const uint32_t len = std::min(interval, steps - seg * interval);
         for (uint32_t t = 0; t < len; ++t) {
             p.store_dv = (seg * interval + t + 1 == steps) ? 1 : 0;
+            // the last segment records (rho, u*) as the reverse pass's recompute would
+            p.rec = seg + 1 == nseg ? (char*)s->rec + (size_t)t * 4 * s->nq * tcs : nullptr;
             if (diss && p.store_dv) {  // the objective is on the final step's collision input
                 StepParams q = p;
+                q.rec = nullptr;
                 q.dseed = s->dseed;
                 q.dscale = 1.0 / (double)s->nfluid;
                 if (s->prec == LB_FP64) launch_diss_rlx<LB_FP64>(s, q, s->next_odd != 0);
@@ -2556,6 +2587,7 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
     s->adj_axis = axis;
     s->adj_scale = 1.0 / n;
     s->adj_ready = true;
+    s->rec_fwd = true;
     return LB_OK;
 }
 
@@ -2566,22 +2598,13 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
         return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", gface_id);
     const size_t tcs = tc_size(s->prec);
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
-    const size_t recb = (size_t)s->adj_interval * 4 * s->nq * tcs;
     int rc;
     if (!s->lam && (rc = dalloc(s, &s->lam, (size_t)s->fsize * tcs))) return rc;
     if (!s->gacc && (rc = dalloc(s, &s->gacc, (size_t)3 * s->nq * tcs))) return rc;
     const size_t nface = (size_t)(s->NI > s->NJ ? s->NI : s->NJ) * (s->NJ > s->NK ? s->NJ : s->NK) * 3;
     if (!s->gface && (rc = dalloc(s, (void**)&s->gface, std::max<size_t>(nface, 3 * (size_t)s->NI * s->NJ) * 8)))
         return rc;
-    if (s->rec_bytes < recb) {
-        if (s->rec) {
-            HIPCHK(hipFree(s->rec));
-            s->bytes -= s->rec_bytes;
-            s->rec = nullptr;
-        }
-        if ((rc = dalloc(s, &s->rec, recb))) return rc;
-        s->rec_bytes = recb;
-    }
+    if ((rc = ensure_rec(s, std::min(s->adj_interval, s->adj_steps)))) return rc;
     const size_t fface = gface_id < 0 ? 0
                          : (size_t)(gface_id < 2 ? s->NJ * s->NK : gface_id < 4 ? s->NI * s->NK : s->NI * s->NJ);
     const unsigned fb = 256;
@@ -2608,14 +2631,22 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     x.obj_scale = s->adj_scale;
     x.dseed = s->dseed;
     for (int seg = (int)nseg - 1; seg >= 0; --seg) {
-        HIPCHK(hipMemcpyAsync(s->f, (char*)s->ckpt + (size_t)seg * fbytes, fbytes, hipMemcpyDeviceToDevice, s->stream));
         const uint32_t g0 = (uint32_t)seg * interval;
         const uint32_t len = std::min(interval, steps - g0);
-        for (uint32_t t = 0; t < len; ++t) {  // recompute, recording (rho, u*)
-            p.rec = (char*)s->rec + (size_t)t * 4 * s->nq * tcs;
-            launch_any_step(s, p, (s->adj_po ^ ((g0 + t) & 1)) != 0);
+        if (seg + 1 == (int)nseg && s->rec_fwd) {
+            // recorded by the forward pass; the populations are the segment's end state
+            HIPCHK(hipMemcpyAsync(s->f, (char*)s->ckpt + (size_t)nseg * fbytes, fbytes, hipMemcpyDeviceToDevice,
+                                  s->stream));
+        } else {
+            s->rec_fwd = false;  // rec now holds an earlier segment
+            HIPCHK(hipMemcpyAsync(s->f, (char*)s->ckpt + (size_t)seg * fbytes, fbytes, hipMemcpyDeviceToDevice,
+                                  s->stream));
+            for (uint32_t t = 0; t < len; ++t) {  // recompute, recording (rho, u*)
+                p.rec = (char*)s->rec + (size_t)t * 4 * s->nq * tcs;
+                launch_any_step(s, p, (s->adj_po ^ ((g0 + t) & 1)) != 0);
+            }
+            p.rec = nullptr;
         }
-        p.rec = nullptr;
         for (int t = (int)len - 1; t >= 0; --t) {
             const uint32_t g = g0 + (uint32_t)t;
             x.rec = (const char*)s->rec + (size_t)t * 4 * s->nq * tcs;
@@ -2655,6 +2686,7 @@ int lb_adj_backward_device(lb_solver* s, int face, double* grad_acc_dev, double*
 
 int lb_set_face_data_device(lb_solver* s, int face, const double* den_dev, const double* vel_dev) {
     if (!s || face < 0 || face > 5 || !den_dev || !vel_dev) return fail(LB_EINVAL, "invalid argument");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     if (!s->fden[face]) return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", face);
     const size_t n = (size_t)(face < 2 ? s->NJ * s->NK : face < 4 ? s->NI * s->NK : s->NI * s->NJ);
     HIPCHK(hipMemcpyAsync(s->fden[face], den_dev, n * 8, hipMemcpyDeviceToDevice, s->stream));
@@ -2680,6 +2712,7 @@ int lb_snapshot(lb_solver* s) {
 
 int lb_restore(lb_solver* s) {
     if (!s) return fail(LB_EINVAL, "null handle");
+    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
     if (!s->snap) return fail(LB_ESTATE, "no snapshot taken");
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
     HIPCHK(hipMemcpyAsync(s->f, s->snap, fbytes, hipMemcpyDeviceToDevice, s->stream));

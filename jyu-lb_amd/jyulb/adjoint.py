@@ -17,7 +17,9 @@ momentum flux of the last step's collision input (the viscous stress is
 -(1 - omega/2) Pi_neq). The reverse pass is the
 hand-written adjoint AA step of libjyulb_hip.so with checkpoints every
 `interval` steps and recomputation (include/jyulb_gpu.h: lb_adj_forward,
-lb_adj_backward). Parameters are held fixed over the steps; the populations at
+lb_adj_backward). The forward pass records its last segment, so with
+interval >= steps nothing is recomputed, at 16 bytes per site and step in FP32
+(32 in FP64). Parameters are held fixed over the steps; the populations at
 the start are the Solver's current state. Walls must be stationary; the
 arithmetic is FP64 or FP32 (the Solver's precision).
 """

@@ -149,6 +149,35 @@ def test_adjoint_periodic_and_checkpoint_invariance():
     assert abs(adjA - fdA) <= 1e-6 * abs(fdA)
 
 
+@pytest.mark.parametrize("layout,obj", [("rows", FACE_X1), ("sparse", OBJ_DISS)])
+def test_adjoint_forward_record_reused_and_invalidated(layout, obj):
+    """The forward pass records (rho, u*) over its last segment and the reverse
+    pass skips that segment's recompute. A second reverse pass (interval <
+    steps: the record was overwritten by an earlier segment) and one after an
+    input is set again (the record is dropped) recompute instead; all agree
+    bit for bit, and with interval >= steps nothing is recomputed."""
+    s, prof, amp = make_channel(TRT, "fp64", seed=6, layout=layout)
+    acc, fden, fvel, g = _inputs(s)
+    eng = s.engine
+    eng.snapshot()
+    ref = None
+    for interval in (29, 6, 40):
+        print(f"adjoint record {layout} obj {obj} interval {interval}", flush=True)
+        J, ga, gf = _adjoint(s, acc, fden, fvel, g, 29, interval, obj)
+        again = eng.adj_backward(FACE_1X)
+        eng.set_face_data(FACE_1X, fden, fvel)
+        fresh = eng.adj_backward(FACE_1X)
+        for a, b in (again, fresh):
+            np.testing.assert_array_equal(a, ga)
+            np.testing.assert_array_equal(b, gf)
+        if ref is None:
+            ref = (J, ga, gf)
+        assert J == ref[0]
+        np.testing.assert_array_equal(ga, ref[1])
+        np.testing.assert_array_equal(gf, ref[2])
+    assert np.max(np.abs(ref[1])) > 0 and np.max(np.abs(ref[2])) > 0
+
+
 def test_adjoint_torch_autograd_function():
     """The torch.autograd.Function returns the engine's gradients."""
     import torch

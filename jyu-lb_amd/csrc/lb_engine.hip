@@ -2517,6 +2517,8 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
     const uint32_t nseg = (steps + interval - 1) / interval;
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
     int rc;
+    s->adj_ready = false;  // until this pass completes: the checkpoints are overwritten
+    s->rec_fwd = false;
     if (s->ckpt_bytes < (size_t)(nseg + 1) * fbytes) {
         if (s->ckpt) {
             HIPCHK(hipFree(s->ckpt));
@@ -2527,7 +2529,6 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         s->ckpt_bytes = (size_t)(nseg + 1) * fbytes;
     }
     const size_t tcs = tc_size(s->prec);
-    s->rec_fwd = false;
     if ((rc = ensure_rec(s, std::min(interval, steps)))) return rc;
     const size_t drows = (size_t)(Q + 4) * s->nq;
     if (diss) {

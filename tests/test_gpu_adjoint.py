@@ -178,6 +178,29 @@ def test_adjoint_forward_record_reused_and_invalidated(layout, obj):
     assert np.max(np.abs(ref[1])) > 0 and np.max(np.abs(ref[2])) > 0
 
 
+def test_adjoint_failed_forward_leaves_no_reverse_pass():
+    """A forward pass that fails after its steps (objective face without a
+    fluid site) has overwritten the checkpoints: the reverse pass of the
+    earlier, successful forward pass is refused afterwards."""
+    from jyulb.isothermal import Solver
+    from jyulb.defs import PERIODIC
+    lat = (8, 6, 5)
+    s = Solver(np.array(lat, dtype=np.int32), BGK, 0.1, np.array([1e-5, 0.0, 0.0]),
+               np.array([PERIODIC] * 6, dtype=np.int32))
+    ph = s.get_phase()
+    ph[...] = 1
+    ph[-1, :, :] = 0  # the x = nx-1 layer is solid
+    s.finalize_init()
+    eng = s.engine
+    eng.adj_forward(6, 4, FACE_1X, X)
+    ga, _ = eng.adj_backward()
+    assert np.max(np.abs(ga)) > 0
+    with pytest.raises(ValueError, match="no fluid site"):
+        eng.adj_forward(6, 4, FACE_X1, X)
+    with pytest.raises(RuntimeError, match="must precede"):
+        eng.adj_backward()
+
+
 def test_adjoint_torch_autograd_function():
     """The torch.autograd.Function returns the engine's gradients."""
     import torch

@@ -44,16 +44,55 @@ def _finish(solver, steps):
     solver.tot_tsteps = (solver.tot_tsteps + int(steps)) & 0xFFFFFFFF
 
 
+def checkpoint_interval(solver, steps, budget_bytes=None):
+    """The largest checkpoint interval whose checkpoints and (rho, u*) record
+    fit in `budget_bytes` (default: half of the free device memory).
+
+    The reverse pass recomputes all but the last segment, so a larger interval
+    is faster; interval = steps recomputes nothing. Memory for interval i:
+    (ceil(steps / i) + 1) population arrays (19 values per site) plus
+    min(i, steps) x 4 values per site, with nx * ny * round_up(nz, 64) sites
+    and 8 bytes per value in FP64, 4 in FP32. Raises MemoryError if even the
+    smallest footprint does not fit.
+    """
+    from ._native import LB_FP64
+    eng = solver.engine
+    nx, ny, nz = eng.lat
+    nq = int(nx) * int(ny) * ((int(nz) + 63) // 64 * 64)
+    vb = 8 if eng.precision == LB_FP64 else 4
+    fb, rb = 19 * nq * vb, 4 * nq * vb
+    steps = int(steps)
+    if steps < 1:
+        raise ValueError('steps must be >= 1')
+    if budget_bytes is None:
+        import torch
+        budget_bytes = torch.cuda.mem_get_info()[0] // 2
+
+    def need(i):
+        return (-(-steps // i) + 1) * fb + i * rb
+
+    for i in range(steps, 0, -1):
+        if need(i) <= budget_bytes:
+            return i
+    raise MemoryError(f'the adjoint of {steps} steps needs at least '
+                      f'{min(need(i) for i in range(1, steps + 1))} bytes; {budget_bytes} available')
+
+
+def _interval(solver, steps, interval):
+    return checkpoint_interval(solver, steps) if interval == 'auto' else int(interval)
+
+
 def flux_gradient(solver, steps, interval=50, face=FACE_X1, axis=X, grad_face=None):
     """Advance `solver` by `steps` steps and return (J, grad_acc, grad_face_vel).
 
     grad_acc has the shape of get_acc(); grad_face_vel has the shape of
     solver.bc_data_vel_face_<grad_face> (None if grad_face is None). The
     density/velocity views hold the state after the steps, as after evolve.
+    interval='auto' takes checkpoint_interval(solver, steps).
     """
     _prepare(solver)
     eng = solver.engine
-    J = eng.adj_forward(steps, interval, face, axis)
+    J = eng.adj_forward(steps, _interval(solver, steps, interval), face, axis)
     ga, gf = eng.adj_backward(-1 if grad_face is None else int(grad_face))
     _finish(solver, steps)
     return J, ga, gf
@@ -63,7 +102,7 @@ def dissipation_gradient(solver, steps, interval=50, grad_face=None):
     """Like flux_gradient, for the dissipation objective."""
     _prepare(solver)
     eng = solver.engine
-    J = eng.adj_forward(steps, interval, OBJ_DISSIPATION, 0)
+    J = eng.adj_forward(steps, _interval(solver, steps, interval), OBJ_DISSIPATION, 0)
     ga, gf = eng.adj_backward(-1 if grad_face is None else int(grad_face))
     _finish(solver, steps)
     return J, ga, gf
@@ -134,13 +173,14 @@ _FN = None
 
 def flux_objective(acc, face_vel, solver, steps, interval=50, face=FACE_X1, axis=X, inlet_face=None):
     """torch entry point: J as a differentiable function of the acceleration
-    field `acc` and (optionally) the Zou-He face velocity data `face_vel`."""
+    field `acc` and (optionally) the Zou-He face velocity data `face_vel`.
+    interval='auto' takes checkpoint_interval(solver, steps)."""
     global _FN
     if _FN is None:
         _FN = _torch_function()
     if face_vel is not None and inlet_face is None:
         inlet_face = fixed_face(solver)
-    return _FN.apply(acc, face_vel, solver, int(steps), int(interval), int(face), int(axis),
+    return _FN.apply(acc, face_vel, solver, int(steps), _interval(solver, steps, interval), int(face), int(axis),
                      -1 if inlet_face is None else int(inlet_face))
 
 

@@ -110,3 +110,27 @@ def test_package_exports():
     from jyulb import adjoint, bcs  # noqa: F401
     assert callable(adjoint.flux_objective)
     assert callable(adjoint.flux_gradient)
+
+
+def test_checkpoint_interval_choice():
+    """interval='auto' of the gradient entry points: the largest interval whose
+    checkpoints and (rho, u*) record fit the budget. Lattice 10 x 4 x 70 has
+    10 * 4 * 128 = 5120 padded sites: a population array is 778,240 bytes in
+    FP64 and one step of record 163,840."""
+    from types import SimpleNamespace
+    from jyulb.adjoint import checkpoint_interval
+    from jyulb._native import LB_FP64, LB_FP32_SHIFTED
+    s64 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64))
+    s32 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP32_SHIFTED))
+    # one segment: 2 arrays + 100 steps of record = 17,940,480 bytes
+    assert checkpoint_interval(s64, 100, 17_940_480) == 100
+    # a byte less: 2 segments need 3 arrays (2,334,720) + i steps, i <= 95
+    assert checkpoint_interval(s64, 100, 17_940_479) == 95
+    # FP32 halves every figure
+    assert checkpoint_interval(s32, 100, 17_940_480 // 2) == 100
+    assert checkpoint_interval(s32, 100, 17_940_480 // 2 - 1) == 95
+    # too small for any interval (the least footprint is 20 segments: 6 arrays + 20 steps)
+    with pytest.raises(MemoryError):
+        checkpoint_interval(s64, 100, 6 * 778_240 + 17 * 163_840)
+    with pytest.raises(ValueError):
+        checkpoint_interval(s64, 0, 1 << 40)

@@ -178,6 +178,21 @@ def test_adjoint_forward_record_reused_and_invalidated(layout, obj):
     assert np.max(np.abs(ref[1])) > 0 and np.max(np.abs(ref[2])) > 0
 
 
+def test_flux_gradient_auto_interval():
+    """interval='auto' takes one segment on a small lattice and gives the
+    gradients of a checkpointed run bit for bit."""
+    from jyulb.adjoint import flux_gradient, checkpoint_interval
+    a, _, _ = make_channel(TRT, "fp32", seed=8)
+    b, _, _ = make_channel(TRT, "fp32", seed=8)
+    assert checkpoint_interval(a, 33) == 33
+    Ja, gaa, gfa = flux_gradient(a, 33, "auto", grad_face=FACE_1X)
+    Jb, gab, gfb = flux_gradient(b, 33, 4, grad_face=FACE_1X)
+    assert Ja == Jb
+    np.testing.assert_array_equal(gaa, gab)
+    np.testing.assert_array_equal(gfa, gfb)
+    np.testing.assert_array_equal(np.asarray(a.get_vel()), np.asarray(b.get_vel()))
+
+
 def test_adjoint_failed_forward_leaves_no_reverse_pass():
     """A forward pass that fails after its steps (objective face without a
     fluid site) has overwritten the checkpoints: the reverse pass of the

@@ -500,7 +500,7 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     }
     TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
     TC vel[3];
-    const TC den = collide<RLX, TC, SH>(f, acc, prm, rho0, vel);
+    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel);
 
     if (p.store_dv) {
         p.den[c] = (double)den;
@@ -724,7 +724,7 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
     }
     TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
     TC vel[3];
-    const TC den = collide<RLX, TC, SH>(f, acc, prm, rho0, vel);
+    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel);
     if (p.store_dv) {
         p.den[c] = (double)den;
         p.vel[c] = (double)vel[0];

@@ -2279,6 +2279,7 @@ struct StepCfg {
     uint64_t xcd_cells;  // see tile_index: 0 = XCD order, UINT64_MAX = round-robin
 };
 constexpr int TUNE_WARM = 2, TUNE_STEPS = 4;
+constexpr float TUNE_MARGIN = 0.02f;
 
 // the XCD tile order is a candidate too on lattices of at least 2^24 cells
 constexpr uint64_t TUNE_XCD_CELLS = 1ull << 24;
@@ -2340,10 +2341,11 @@ static int tune_step(lb_solver* s) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(ev[2 * n - 1]));
     int best = 0;
-    float tbest = 0.0f;
+    float tbest = 0.0f, tdef = 0.0f;
     for (int i = 0; i < n; ++i) {
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        if (i == 0) tdef = ms;
         if (i == 0 || ms < tbest) {
             best = i;
             tbest = ms;
@@ -2353,6 +2355,11 @@ static int tune_step(lb_solver* s) {
                     c[i].lds_even, c[i].lds_odd, c[i].xcd_cells ? "round-robin" : "XCD", ms / TUNE_STEPS);
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    // a few timed steps cannot separate near-ties: the default stays unless
+    // beaten by more than TUNE_MARGIN (on the 768x384x384 medium in FP16 the
+    // two tile orders timed within 0.5% here but sustained 37.6k against 35.0k
+    // MLUPS over 500 steps, in favour of the default)
+    if (tbest > (1.0f - TUNE_MARGIN) * tdef) best = 0;
     s->step_tile = c[best].tile;
     s->step_lds[0] = c[best].lds_even;
     s->step_lds[1] = c[best].lds_odd;

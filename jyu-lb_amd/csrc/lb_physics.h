@@ -62,8 +62,8 @@ LBD void moments_h(const T* f, T* h) {
 }
 
 // Kinetic projection P(h): h_to_f (D3Q19.pyx:460-527) when ADD is false,
-// add_h_to_f (D3Q19.pyx:532-599) when true. DIV divides by 72 and 3 instead
-// of multiplying by their rounded reciprocals.
+// add_h_to_f (D3Q19.pyx:532-599) when true. DIV corrects the products with
+// the rounded reciprocals of 72 and 3 to the quotients.
 template <class T, bool ADD, bool DIV = false>
 LBD void project(const T* h, T* f) {
     T e_xy = T(2) * h[0] + T(6) * h[4] + T(6) * h[7] - T(3) * h[9];
@@ -85,9 +85,21 @@ LBD void project(const T* h, T* f) {
     T e_1y = T(4) * h[0] - T(6) * h[4] + T(12) * h[7] - T(6) * h[9];
     T e_1z = T(4) * h[0] - T(6) * h[4] - T(6) * h[7] + T(12) * h[9];
     // In FP32 the rounded 1/72 and 1/3 are 7.5e-9 and 3e-8 too large, a bias
-    // that every step applies to every moment; correctly rounded division
-    // has none. FP64 keeps the reference's multiplications.
-    auto c72 = [](T v) { return DIV ? v / T(72.0) : (T(1.0) / T(72.0)) * v; };
+    // that every step applies to every moment. DIV removes it: the product
+    // with the rounded reciprocal is corrected by its remainder v - q d, which
+    // an FMA computes exactly (one Newton step on the quotient). FP64 keeps
+    // the reference's multiplications.
+    auto quo = [](T v, T d) -> T {
+        const T r = T(1.0) / d;
+        if constexpr (!DIV) {
+            return r * v;
+        } else {
+            static_assert(sizeof(T) == 4, "DIV is an FP32 form");
+            const float q = v * r;
+            return __builtin_fmaf(__builtin_fmaf(-q, d, v), r, q);
+        }
+    };
+    auto c72 = [&](T v) { return quo(v, T(72.0)); };
     T p[19];
     p[0] = c72(e_1x_1y + o_1x_1y);
     p[18] = c72(e_1x_1y - o_1x_1y);
@@ -107,7 +119,7 @@ LBD void project(const T* h, T* f) {
     p[11] = c72(e_1y_z1 - o_1y_z1);
     p[8] = c72(e_1z - T(12) * h[3]);
     p[10] = c72(e_1z + T(12) * h[3]);
-    p[9] = (DIV ? h[0] / T(3.0) : (T(1.0) / T(3.0)) * h[0]) - T(0.5) * (h[4] + h[7] + h[9]);
+    p[9] = quo(h[0], T(3.0)) - T(0.5) * (h[4] + h[7] + h[9]);
 #pragma unroll
     for (int l = 0; l < Q; ++l) f[l] = ADD ? f[l] + p[l] : p[l];
 }
@@ -223,9 +235,9 @@ LBD T collide_reg(T* f, const T* acc, const T* prm, T rho0, T* vel) {
     return den;
 }
 
-// DIV: the projection divides (the FP32-shifted variant: its error against
-// FP64 on a 512x128x128 duct drops from 2.4e-5 to 1.2e-6; FP16 storage,
-// whose error is its storage, keeps the faster multiplications)
+// DIV: the projection's quotients are corrected (the FP32-shifted variant:
+// its error against FP64 on a 512x128x128 duct drops from 2.4e-5 to 1.2e-6;
+// FP16 storage, whose error is its storage, keeps the plain products)
 template <int RLX, class T, bool SH, bool DIV = false>
 LBD T collide(T* f, const T* acc, const T* prm, T rho0, T* vel) {
     if constexpr (RLX == BGK) return collide_bgk<T, SH, DIV>(f, acc, prm, rho0, vel);

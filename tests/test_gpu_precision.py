@@ -92,10 +92,11 @@ def test_duct_full_size_precision():
     """The precision workload at full size: 512 x 128 x 128 duct, regularized
     collision, tau = 0.55, 5000 steps, relative L2 velocity error against
     FP64. FP32 storage with FP64 arithmetic: <= 1e-6. FP16 storage: <= 3e-3.
-    FP32 arithmetic: <= 1e-5 (the stated bar; 1.2e-6 measured). The
-    projection divides by 72 and 3 in FP32: the rounded reciprocals, 7.5e-9
-    and 3e-8 too large, scaled every moment on every step and gave 2.4e-5,
-    growing about linearly with the step count."""
+    FP32 arithmetic: <= 1e-5 (the stated bar; 1.2e-6 measured). The FP32
+    projection corrects its products with the rounded reciprocals of 72 and
+    3 (7.5e-9 and 3e-8 too large) by their exact remainders: uncorrected, they
+    scaled every moment on every step and gave 2.4e-5, growing about linearly
+    with the step count."""
     fields = {}
     for prec in ("fp64", "fp32", "fp16", "fp32_storage"):
         print(f"full duct {prec}", flush=True)

@@ -23,6 +23,7 @@
 #ifndef JYULB_GPU_H
 #define JYULB_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -49,6 +50,9 @@ extern "C" {
                                    the 18 odd-step locations per site (the fluid-only AA arrays of
                                    include/kernel/kernel.cpp:227-271), built on the device and read
                                    as 8-bit codes from per-wave bases; allocated by lb_finalize */
+#define LB_LAYOUT_AUTO 4        /* chosen at the first lb_finalize: SPARSE when at least 5% of the cells are
+                                   solid (LB_AUTO_SPARSE_SOLID in the environment sets the fraction), ROWS
+                                   otherwise; results are identical in every layout */
 
 typedef struct lb_solver lb_solver;
 
@@ -107,8 +111,50 @@ int lb_finalize(lb_solver *s, int64_t *nfluid);
 
 /* The step loop of Solver.evolve (isothermal.pyx:1408-1415): `steps` fused
  * AA stream+collide steps. den/vel of the fluid sites are stored on the last
- * step. Asynchronous on the handle's stream. */
+ * step. Asynchronous on the handle's stream, except that the FP16 storage
+ * variant checks its stored magnitudes (and rescales them by a power of two)
+ * at the start and every 64 steps (LB_FP16_CHECK), failing with LB_ESTATE if
+ * a population overflowed. A recorded adjoint forward pass no longer applies
+ * afterwards (see lb_adj_backward). */
 int lb_evolve(lb_solver *s, uint32_t steps);
+
+/* Pinned host memory (hipHostMalloc) for the Solver's own arrays, so that
+ * the copies of lb_evolve_host run at full DMA speed; lb_host_free releases
+ * it. */
+int lb_host_alloc(size_t bytes, void **ptr);
+int lb_host_free(void *ptr);
+
+/* The Solver's halo-padded host fields (isothermal.pyx:243-248): phase u8
+ * [NI][NJ][NK], den f64 [NI][NJ][NK], vel and acc f64 [NI][NJ][NK][3] with
+ * NI = nx + 2, NJ = ny + 2, NK = nz + 2. lb_evolve_host reads and writes them
+ * in place. Any host memory works; lb_host_alloc memory is faster. They must
+ * stay valid while bound; four NULLs unbind them. Allocates a device copy of
+ * the four fields. */
+int lb_bind_host_fields(lb_solver *s, uint8_t *phase, double *den, double *vel, double *acc);
+
+/* Solver.evolve on the bound host fields (isothermal.pyx:1370-1418), after
+ * the caller has refreshed the Zou-He data (_set_bc_data, lb_set_face_data)
+ * and gravity. Per call: the host fields are copied to the device as they
+ * are; a changed fluid/solid state of any site fails with LB_ESTATE (the
+ * reference re-reads phase every step, the engine builds its links once); the
+ * acceleration field, the wall velocities (moving walls re-derived when they
+ * changed) and the densities Ladd's term reads in the first step come from
+ * them; then `steps` steps; then den/vel of the fluid sites are written back
+ * into the host fields, with the den/vel halo pass of periodic and symmetric
+ * faces (_enforce_den_vel_per_sym_bc, isothermal.pyx:582-686) applied to the
+ * state of step steps-1, as the reference leaves it. SOLID sites keep their
+ * host values. Synchronous. */
+int lb_evolve_host(lb_solver *s, uint32_t steps);
+
+/* The first part of lb_evolve_host alone: the bound host fields taken as the
+ * inputs of the next steps (phase check, acceleration field, wall
+ * velocities, Ladd densities), e.g. before lb_adj_forward. Synchronous. */
+int lb_load_host_fields(lb_solver *s);
+
+/* den/vel of the fluid sites of the last stored step into the bound host
+ * fields (SOLID sites and halos keep their host values), followed by the
+ * den/vel halo pass if halo_pass != 0. Synchronous. */
+int lb_store_host_fields(lb_solver *s, int halo_pass);
 
 /* Re-derive the moving-wall flags after lb_upload_den_vel changed wall
  * velocities (called by the Python Solver before evolve when needed). */
@@ -126,6 +172,21 @@ int lb_download_post_f(lb_solver *s, double *f);
  * out[7] = max |vel|. (The quantities of calc_flow_info,
  * include/solver/solver.cpp:64-129, and of the examples' mass checks.) */
 int lb_reduce(lb_solver *s, double out[8]);
+
+/* calc_flow_info (include/solver/solver.cpp:64-129; the fields of
+ * IsothermalFlowInfo, include/solver/solver.h:13-15) over the fluid sites of
+ * the last stored den/vel, in FP64 on the device for every precision
+ * variant: totals of momentum (j = den u), of |j| = den |u| (tot_j2) and of
+ * density; averages of u and of |u| (ave_u2); max |u| (max_u2) and the
+ * density extrema, which start from the reference density rho0 as in the
+ * reference. fluid_sites (an addition) is the number of sites summed. */
+typedef struct {
+    double tot_jx, tot_jy, tot_jz, tot_j2, tot_den;
+    double ave_ux, ave_uy, ave_uz, ave_u2;
+    double max_u2, max_den, min_den;
+    double fluid_sites;
+} lb_flow_info_t;
+int lb_flow_info(lb_solver *s, lb_flow_info_t *info);
 
 /* Steady-state test in FP64: ||u - u_prev||_2 / ||u||_2 over the fluid
  * sites, then u_prev := u. The first call returns 1.0. */
@@ -168,7 +229,9 @@ int lb_adj_forward(lb_solver *s, uint32_t steps, uint32_t interval, int face, in
  * the forward pass's record instead, unless a call since then changed an input
  * or an earlier reverse pass overwrote it (interval < steps). With interval >=
  * steps nothing is recomputed. The population array is left in its state after
- * the forward pass. Null outputs are skipped. */
+ * the forward pass. Null outputs are skipped. Refused (LB_ESTATE) unless it
+ * follows its forward pass directly: an evolve step, an input upload, a
+ * finalize or a restore in between invalidates the forward pass. */
 int lb_adj_backward(lb_solver *s, int face, double *grad_acc, double *grad_face);
 int lb_adj_backward_device(lb_solver *s, int face, double *grad_acc_dev, double *grad_face_dev);
 
@@ -181,6 +244,10 @@ int lb_restore(lb_solver *s);
 /* Introspection: number of AA steps taken, parity of the next step (1 = odd),
  * bytes of device memory held, storage scale of the FP16 variant. */
 int lb_info(lb_solver *s, uint64_t *steps, int *next_odd, uint64_t *device_bytes, double *fp16_scale);
+
+/* The population layout in use (LB_LAYOUT_*; LB_LAYOUT_AUTO until the first
+ * lb_finalize resolves it). */
+int lb_layout(lb_solver *s, int *layout);
 
 /* Synchronise the handle's stream. */
 int lb_synchronize(lb_solver *s);

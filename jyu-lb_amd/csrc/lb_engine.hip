@@ -20,6 +20,11 @@
 // and written by one thread only, so one array suffices (the C++ exe of the
 // reference uses the same scheme with a tabulated index map,
 // include/kernel/kernel.cpp:94-139, 469-552; here addresses are computed).
+//
+// This translation unit holds the host side (the C ABI, solver state, launch
+// tuning) and the set-up, import/export and reduction kernels; the fused step,
+// adjoint and initialisation kernels are built per storage variant from
+// lb_kernels.h (lb_inst.hip).
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <math.h>
@@ -32,10 +37,7 @@
 #include <string.h>
 #include <string>
 
-#include "../../include/jyulb_gpu.h"
-#include "lb_lattice.h"
-#include "lb_physics.h"
-#include "lb_adjoint.h"
+#include "lb_common.h"
 
 using namespace lb;
 
@@ -61,904 +63,6 @@ static int fail(int code, const char* fmt, ...) {
             return fail(LB_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
                         __FILE__, __LINE__);                                             \
     } while (0)
-
-// ---------------------------------------------------------------------------
-// storage variants
-// ---------------------------------------------------------------------------
-template <int P> struct Var;
-template <> struct Var<LB_FP64> { using TS = double; using TC = double; static constexpr bool SH = false; };
-template <> struct Var<LB_FP32_SHIFTED> { using TS = float; using TC = float; static constexpr bool SH = true; };
-template <> struct Var<LB_FP16_STORAGE> { using TS = _Float16; using TC = float; static constexpr bool SH = true; };
-template <> struct Var<LB_FP32_STORAGE> { using TS = float; using TC = double; static constexpr bool SH = true; };
-
-template <class TC> LBD TC ldp(const double* p) { return (TC)(*p); }
-
-// Cache policy of the population loads and stores: every population is
-// touched once per step, so streaming (non-temporal) accesses keep L2 for the
-// lines that neighbouring waves share. Separate switches per step parity and
-// direction (bit 0 even loads, 1 even stores, 2 odd loads, 3 odd stores;
-// bit 4: in odd steps stream the populations without a k component and
-// apply bits 2/3 only to the k-shifted ones).
-#ifndef LB_NT_MASK
-#define LB_NT_MASK 3
-#endif
-template <bool NT, class T> LBD T ld_stream(const T* p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-template <bool NT, class T> LBD void st_stream(T* p, T v) {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
-
-template <int P, bool NT>
-LBD typename Var<P>::TC ldf(const typename Var<P>::TS* p, float inv_scale) {
-    if constexpr (P == LB_FP16_STORAGE) {
-        return (float)ld_stream<NT>(p) * inv_scale;
-    } else {
-        (void)inv_scale;
-        return ld_stream<NT>(p);
-    }
-}
-template <int P, bool NT>
-LBD void stf(typename Var<P>::TS* p, typename Var<P>::TC v, float scale) {
-    if constexpr (P == LB_FP16_STORAGE)
-        st_stream<NT>(p, (_Float16)(v * scale));
-    else {
-        (void)scale;
-        st_stream<NT>(p, (typename Var<P>::TS)v);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// geometry helpers
-// ---------------------------------------------------------------------------
-struct Dims {
-    int nx, ny, nz, nzp;
-    int bc[6];
-};
-
-__constant__ int8_t c_mirror[3][Q];
-
-// Resolve a possibly out-of-range site (i,j,k) reached along lattice vector l
-// to the interior site whose data it carries, undoing the reference's halo
-// passes in reverse order (z, then y, then x: the x pass covers interior j,k
-// only, the y pass all i and interior k, the z pass all i and j,
-// isothermal.pyx:509-837). Periodic faces wrap, symmetric faces mirror the
-// site onto the adjacent layer and reflect l and the velocity component
-// (flip bit per axis). Returns false for a WALL/FIXED halo: a SOLID site at
-// rest.
-LBD bool resolve(const Dims& d, int& i, int& j, int& k, int& l, int& flip) {
-    int c[3] = {i, j, k};
-    const int n[3] = {d.nx, d.ny, d.nz};
-#pragma unroll
-    for (int a = 2; a >= 0; --a) {
-        if (c[a] < 0 || c[a] >= n[a]) {
-            const bool lo = c[a] < 0;
-            const int t = d.bc[2 * a + (lo ? 0 : 1)];
-            if (t == PERIODIC) {
-                c[a] += lo ? n[a] : -n[a];
-            } else if (t == SYMMETRIC) {
-                c[a] = lo ? 0 : n[a] - 1;
-                l = c_mirror[a][l];
-                flip ^= 1 << a;
-            } else {
-                return false;
-            }
-        }
-    }
-    i = c[0];
-    j = c[1];
-    k = c[2];
-    return true;
-}
-
-// A pointer known to be equal across the wave. Marking it keeps the compiler
-// from folding per-lane offsets into it early: the address of a load then
-// stays "scalar base + 32-bit lane offset" instead of one 64-bit vector
-// address per population held in registers until the store.
-template <class T>
-LBD T* uniform_ptr(T* p) {
-    const uint64_t v = (uint64_t)p;
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return (T*)(((uint64_t)hi << 32) | lo);
-}
-
-LBD void decompose(uint32_t c, int ny, int nzp, int& i, int& j, int& k) {
-    uint32_t row = c / (uint32_t)nzp;
-    k = (int)(c - row * (uint32_t)nzp);
-    i = (int)(row / (uint32_t)ny);
-    j = (int)(row - (uint32_t)i * (uint32_t)ny);
-}
-
-// ---------------------------------------------------------------------------
-// step kernel
-// ---------------------------------------------------------------------------
-// code type of the sparse layout's compact odd-step table (see step_sparse):
-// 8 bits measured 4-6% faster odd steps than 16 on the 768x384x384 porous
-// medium; locations beyond an 8-bit offset fall back to the 32-bit table
-#ifndef LB_SP_BITS
-#define LB_SP_BITS 8
-#endif
-#if LB_SP_BITS == 8
-typedef uint8_t SpCode;
-#else
-typedef uint16_t SpCode;
-#endif
-
-struct StepParams {
-    void* f;
-    const uint32_t* mask;
-    double* den;
-    double* vel;  // 3 * nq, SoA
-    const void* acc;  // 3 * nq of the compute type, SoA, or null
-    const double* fden[6];
-    const double* fvel[6];
-    double g[3];
-    double prm[2];
-    double rho0;
-    float scale, inv_scale;
-    Dims d;
-    int NJ, NK;  // halo-padded sizes (face-data indexing)
-    uint32_t nq;
-    uint32_t ntiles;
-    int store_dv;
-    int moving;
-    int64_t rs;  // element stride between rows (i,j) of the population array
-    int64_t ps;  // element stride between populations
-    const uint8_t* wclass;  // per wave of 64 sites: 0 = plain (see k_mask)
-    uint32_t xcd;           // deal tiles to XCDs in contiguous ranges (tile_index)
-    uint32_t lds[2];        // dynamic LDS per workgroup of the bulk step launch (even, odd): caps occupancy
-    uint32_t tile;          // threads per workgroup of the launch that reads ntiles (see TILE)
-    const uint32_t* mov_idx;  // per site: index of a moving site in ladd_u
-    const double* ladd_u;     // per moving site: the 18 wall-velocity sums of Ladd's term
-    uint32_t nwaves;
-    // sites with M_SPECIAL: those on a Zou-He face first (nzh of them), then
-    // the rest; ascending within each wave
-    const uint32_t* special;
-    uint32_t nspecial, nzh;
-    void* rec;  // if set: (rho, u*) of this step, 4 * nq of the compute type (adjoint recompute)
-    // sparse layout (LB_LAYOUT_SPARSE), null otherwise: compact index of every
-    // site (fluid sites only), site of every compact index, and the odd-step
-    // location of link l of compact site n at tab[link_slot(l) * ps + n]
-    const uint32_t* cidx;
-    const uint32_t* site;
-    const uint32_t* tab;
-    const SpCode* tabc;     // compact form of tab (see step_sparse)
-    const uint32_t* tabw;   // its base locations, 18 per wave of 64 compact sites
-    uint32_t nf;  // fluid sites = compact sites the step processes
-    // dissipation objective (k_diss): per site 19 population seeds, 3
-    // acceleration seeds and the site's Pi^neq:Pi^neq, each an nq-long row of
-    // the compute type; dscale = 1 / (fluid sites)
-    void* dseed;
-    double dscale;
-};
-
-// expanded index of padded site c in the population array: population l of
-// site (i,j,k) lives at (i*ny + j)*rs + k + l*ps
-LBD int64_t eidx(const StepParams& p, uint32_t c) {
-    if (p.cidx) return p.cidx[c];  // sparse: population l of site c at cidx[c] + l*ps
-    if (p.rs == p.d.nzp) return c;
-    const uint32_t row = c / (uint32_t)p.d.nzp;
-    return (int64_t)row * p.rs + (c - row * (uint32_t)p.d.nzp);
-}
-
-// Threads per workgroup: TILE for the adjoint and special-site kernels; the
-// bulk step kernels run p.tile threads (a multiple of 64, at most
-// STEP_TILE_MAX), chosen per precision at lb_create.
-constexpr int TILE = 256;
-constexpr int STEP_TILE_MAX = 512;
-constexpr int RED_BLOCK_ADJ = 256;
-
-// element offset of the location an odd step reads f_l from (and writes
-// f*_opp(l) back to)
-template <bool SPECIAL = true>
-LBD int64_t odd_loc(const StepParams& p, uint32_t m, int64_t e, int l, int i, int j, int k) {
-    if (l == REST) return e + (int64_t)REST * p.ps;
-    if (m & (1u << l)) return e + (int64_t)l * p.ps;
-    if (!SPECIAL || !(m & (M_BND | M_KWRAP)))
-        return e - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs - CK[l] + (int64_t)opp(l) * p.ps;
-    int yi = i - CI[l], yj = j - CJ[l], yk = k - CK[l], lp = l, fl = 0;
-    resolve(p.d, yi, yj, yk, lp, fl);
-    return ((int64_t)yi * p.d.ny + yj) * p.rs + yk + (int64_t)(18 - lp) * p.ps;
-}
-
-// Ladd's moving-wall term for bounce-back link l (isothermal.pyx:1112-1248):
-// f_l = f_opp +/- cff*(u_a +/- u_b), u the velocity of the solid site
-// x - c_l (mirrored through symmetric faces, zero for WALL/FIXED halos), a and
-// b the non-zero components of c_l, cff = 2 rho_prev w_l 3. The velocity sum
-// (u_a, or u_a +/- u_b for diagonal links) depends only on the walls, so it is
-// tabulated per moving site (k_ladd_prep, 18 links) whenever the walls change;
-// the step kernel adds cff * sum with the sign of c_l's first component.
-template <int L>
-LBD double ladd_sum(const Dims& d, const double* vel, int64_t nq, int i, int j, int k) {
-    int yi = i - CI[L], yj = j - CJ[L], yk = k - CK[L], lp = L, fl = 0;
-    double u[3] = {0.0, 0.0, 0.0};
-    if (resolve(d, yi, yj, yk, lp, fl)) {
-        const int64_t y = ((int64_t)yi * d.ny + yj) * d.nzp + yk;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            double v = vel[(int64_t)a * nq + y];
-            u[a] = (fl >> a & 1) ? -v : v;
-        }
-    }
-    constexpr int c[3] = {CI[L], CJ[L], CK[L]};
-    constexpr int a = c[0] != 0 ? 0 : (c[1] != 0 ? 1 : 2);
-    constexpr int b = nnz(L) == 1 ? -1 : (a == 0 ? (c[1] != 0 ? 1 : 2) : 2);
-    if constexpr (b < 0) return u[a];
-    else if constexpr (c[a] == c[b]) return u[a] + u[b];
-    else return u[a] - u[b];
-}
-
-// table slot of link l (the rest population has none)
-LBD constexpr int link_slot(int l) { return l < REST ? l : l - 1; }
-// sign of the first non-zero component of c_l
-LBD constexpr bool ladd_plus(int l) { return (CI[l] != 0 ? CI[l] : CJ[l] != 0 ? CJ[l] : CK[l]) > 0; }
-
-template <class T>
-LBD void ladd_apply(const StepParams& p, uint32_t c, uint32_t m, T* f) {
-    const T den = (T)p.den[c];
-    const T cff1 = T(2.0) * den * T(2.0 / 36.0) * T(3.0);
-    const T cff2 = T(2.0) * den * T(1.0 / 36.0) * T(3.0);
-    const double* U = p.ladd_u + (int64_t)p.mov_idx[c] * (Q - 1);
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        if (l == REST || !(m & (1u << l))) continue;
-        const T term = (nnz(l) == 1 ? cff1 : cff2) * (T)U[link_slot(l)];
-        f[l] = ladd_plus(l) ? f[l] + term : f[l] - term;
-    }
-}
-
-// Build switches of the bulk kernels. LB_PLAIN_PATH=1: waves whose 64 sites
-// are all plain fluid (wave class 0) take the path that loads no link mask;
-// 0 sends every wave through the mask path. LB_WAVES_PER_EU > 0 asks the
-// compiler for that many waves per SIMD in k_step (0: its own choice; 5 or 6
-// for the FP64 kernels spill registers and measured about half the speed).
-#ifndef LB_PLAIN_PATH
-#define LB_PLAIN_PATH 1
-#endif
-#ifndef LB_WAVES_PER_EU
-#define LB_WAVES_PER_EU 0
-#endif
-#if LB_WAVES_PER_EU > 0
-#define LB_OCC __attribute__((amdgpu_waves_per_eu(LB_WAVES_PER_EU, 8)))
-#else
-#define LB_OCC
-#endif
-
-// One site of the fused step, in one of three modes:
-//   MODE_PLAIN    the site and its whole wave are interior fluid sites with no
-//                 bounce-back link: no link mask, loads issue immediately
-//   MODE_BULK     ordinary fluid site: bounce-back links from the link mask,
-//                 including Ladd's moving-wall term
-//   MODE_SPECIAL  sites with halo-crossing links (periodic / symmetric faces)
-//                 or on a Zou-He face; processed by a separate
-//                 kernel over a compact list so that these rarely taken paths
-//                 do not raise the register budget of the bulk kernel
-constexpr int MODE_PLAIN = 0, MODE_BULK = 1, MODE_SPECIAL = 2;
-
-// Zou-He faces of site (i,j,k), in the order 1X, X1, 1Y, Y1, 1Z, Z1
-// (isothermal.pyx:1280-1291), on the streamed populations f
-template <class TC, bool SH>
-LBD void zouhe_site(const StepParams& p, uint32_t m, int i, int j, int k, TC rho0, TC* f) {
-#pragma unroll
-    for (int fc = 0; fc < 6; ++fc) {
-        if (!(m & (M_ZH0 << fc))) continue;
-        int64_t fi;
-        if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
-        else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
-        else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
-        TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
-        if (p.d.bc[fc] == FIXED_DEN) zh_fix_den<TC, SH>(fc, (TC)p.fden[fc][fi], vb, rho0, f);
-        else zh_fix_vel<TC, SH>(fc, vb, rho0, f);
-    }
-}
-
-// Dissipation objective J = (1/N) sum over the fluid sites of Pi^neq:Pi^neq,
-// the non-equilibrium momentum flux of the final step's collision input f
-// (streamed, bounce-back and Zou-He applied): Pi^neq = sum_l c_l c_l f_l -
-// rho u* u* - rho/3 I with u* = j/rho + a/2. Writes the site's term and the
-// seeds dJ/df_l = (2/N) [c_l.Pi.c_l - 2 (c_l + a/2).Pi.u* + u*.Pi.u* -
-// tr(Pi)/3] and dJ/da = -(2/N) rho Pi.u* (Pi = Pi^neq, symmetric).
-template <class TC, bool SH>
-LBD void diss_site(const StepParams& p, const uint32_t c, const TC* f, const TC* acc) {
-    TC s0 = TC(0), j[3] = {TC(0), TC(0), TC(0)}, Pi[6] = {TC(0), TC(0), TC(0), TC(0), TC(0), TC(0)};
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const TC cx = TC(CI[l]), cy = TC(CJ[l]), cz = TC(CK[l]);
-        s0 += f[l];
-        j[0] += cx * f[l];
-        j[1] += cy * f[l];
-        j[2] += cz * f[l];
-        Pi[0] += cx * cx * f[l];
-        Pi[1] += cx * cy * f[l];
-        Pi[2] += cx * cz * f[l];
-        Pi[3] += cy * cy * f[l];
-        Pi[4] += cy * cz * f[l];
-        Pi[5] += cz * cz * f[l];
-    }
-    const TC rho0 = (TC)p.rho0, third = TC(1.0) / TC(3.0);
-    const TC rho = SH ? rho0 + s0 : s0;
-    if (SH) {  // shifted populations: sum_l w_l c_l c_l rho0 = rho0/3 I
-        Pi[0] += rho0 * third;
-        Pi[3] += rho0 * third;
-        Pi[5] += rho0 * third;
-    }
-    TC u[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) u[a] = j[a] / rho + TC(0.5) * acc[a];
-    const TC N[6] = {Pi[0] - rho * u[0] * u[0] - rho * third, Pi[1] - rho * u[0] * u[1], Pi[2] - rho * u[0] * u[2],
-                     Pi[3] - rho * u[1] * u[1] - rho * third, Pi[4] - rho * u[1] * u[2],
-                     Pi[5] - rho * u[2] * u[2] - rho * third};
-    const TC Nu[3] = {N[0] * u[0] + N[1] * u[1] + N[2] * u[2], N[1] * u[0] + N[3] * u[1] + N[4] * u[2],
-                      N[2] * u[0] + N[4] * u[1] + N[5] * u[2]};
-    const TC uNu = u[0] * Nu[0] + u[1] * Nu[1] + u[2] * Nu[2];
-    const TC tr = N[0] + N[3] + N[5];
-    const TC sc = TC(2.0) * (TC)p.dscale;
-    TC* S = (TC*)p.dseed;
-    const int64_t nq = p.nq;
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        const TC cx = TC(CI[l]), cy = TC(CJ[l]), cz = TC(CK[l]);
-        const TC cNc = cx * (N[0] * cx + N[1] * cy + N[2] * cz) + cy * (N[1] * cx + N[3] * cy + N[4] * cz) +
-                       cz * (N[2] * cx + N[4] * cy + N[5] * cz);
-        const TC wNu =
-            (cx + TC(0.5) * acc[0]) * Nu[0] + (cy + TC(0.5) * acc[1]) * Nu[1] + (cz + TC(0.5) * acc[2]) * Nu[2];
-        S[(int64_t)l * nq + c] = sc * (cNc - TC(2.0) * wNu + uNu - tr * third);
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) S[(int64_t)(Q + a) * nq + c] = -sc * rho * Nu[a];
-    S[(int64_t)(Q + 3) * nq + c] =
-        N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + TC(2.0) * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
-}
-
-template <int P, int RLX, bool ODD, int MODE, bool DISS = false>
-LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
-    constexpr bool PLAIN = MODE != MODE_SPECIAL;  // no special handling needed
-    using TS = typename Var<P>::TS;
-    using TC = typename Var<P>::TC;
-    constexpr bool SH = Var<P>::SH;
-    TS* F = (TS*)p.f;
-    const int64_t nq = p.nq;
-    const int64_t e = eidx(p, c);
-    int i = 0, j = 0, k = 0;
-    if (MODE == MODE_SPECIAL) decompose(c, p.d.ny, p.d.nzp, i, j, k);
-
-    // Bulk sites address populations as a wave-uniform row base plus the
-    // unsigned lane offset kk (a wave never straddles rows: nzp % 64 == 0), so
-    // every load and store uses a scalar base and one shared 32-bit offset.
-    // Bounce-back links are exec-masked loads from the site's own slot.
-    const uint32_t nzp = (uint32_t)p.d.nzp;
-    const uint32_t row = PLAIN ? __builtin_amdgcn_readfirstlane(c / nzp) : 0;
-    const uint32_t kk = c - row * nzp;
-    TS* const Fr = F + (int64_t)row * p.rs;
-    auto own = [&](int l) { return uniform_ptr(Fr + (int64_t)l * p.ps); };
-    auto up = [&](int l) {
-        return uniform_ptr(Fr - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs + (int64_t)opp(l) * p.ps - CK[l]);
-    };
-    // z-periodic wrap of link l for this lane: the upstream element is the
-    // other end of the same upstream row (a uniform address)
-    const uint32_t nzm = (uint32_t)p.d.nz - 1u;
-    auto wraps = [&](int l) {
-        return MODE == MODE_BULK && (m & M_KWRAP) && ((CK[l] > 0 && kk == 0) || (CK[l] < 0 && kk == nzm));
-    };
-    auto upw = [&](int l) { return up(l) + CK[l] + (CK[l] > 0 ? nzm : 0u); };
-    // wave-uniform: does any lane wrap? Other waves run the pull and store
-    // loops without the per-lane wrap tests.
-    const bool anyw = MODE == MODE_BULK && ODD && __ballot((m & M_KWRAP) != 0) != 0;
-    constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
-    constexpr bool NTLO = (LB_NT_MASK & 4) != 0, NTSO = (LB_NT_MASK & 8) != 0;
-
-    // ---- stream (pull) ----
-    TC f[Q];
-    if constexpr (PLAIN && !ODD) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l) + kk, p.inv_scale);
-    } else if constexpr (PLAIN) {
-        auto pull = [&](auto wrap) {
-            constexpr bool WRAP = decltype(wrap)::value;
-#pragma unroll
-            for (int l = 0; l < Q; ++l) {
-                // k-shifted pulls straddle cache lines shared with the next wave:
-                // LB_NT_MASK bit 4 keeps those cached and streams the rest
-                constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTLO;
-                if (l == REST) f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
-                else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
-                    if (CK[l] == 0) f[l] = ldf<P, NTK>(up(l) + kk, p.inv_scale);
-                    else if (WRAP && wraps(l)) f[l] = ldf<P, NTLO>(upw(l), p.inv_scale);
-                    else f[l] = ldf<P, NTLO>(up(l) + kk, p.inv_scale);
-                } else f[l] = ldf<P, NTK>(own(l) + kk, p.inv_scale);
-            }
-        };
-        if (anyw) pull(std::true_type{});
-        else pull(std::false_type{});
-    } else if constexpr (!ODD) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(F + e + (int64_t)l * p.ps, p.inv_scale);
-    } else {
-#pragma unroll
-        for (int l = 0; l < Q; ++l)
-            f[l] = ldf<P, NTLO>(F + odd_loc<true>(p, m, e, l, i, j, k), p.inv_scale);
-    }
-    const TC rho0 = (TC)p.rho0;
-
-    // ---- Ladd moving walls on bounce-back links (previous-step density) ----
-    if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) ladd_apply<TC>(p, c, m, f);
-
-    // ---- Zou-He faces ----
-    if (!PLAIN && (m & M_ZH_ANY)) zouhe_site<TC, SH>(p, m, i, j, k, rho0, f);
-
-    // ---- acceleration and collision ----
-    TC acc[3];
-    const TC* A = (const TC*)p.acc;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
-    if constexpr (DISS) {  // objective gather: nothing is stored to f
-        diss_site<TC, SH>(p, c, f, acc);
-        return;
-    }
-    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
-    TC vel[3];
-    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel);
-
-    if (p.store_dv) {
-        p.den[c] = (double)den;
-        p.vel[c] = (double)vel[0];
-        p.vel[nq + c] = (double)vel[1];
-        p.vel[2 * nq + c] = (double)vel[2];
-    } else if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) {
-        p.den[c] = (double)den;
-    }
-    if (p.rec) {
-        TC* R = (TC*)p.rec;
-        R[c] = den;
-        R[nq + c] = vel[0];
-        R[2 * nq + c] = vel[1];
-        R[3 * nq + c] = vel[2];
-    }
-
-    // ---- store ----
-    if constexpr (PLAIN && !ODD) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)) + kk, f[l], p.scale);
-    } else if constexpr (PLAIN) {
-        auto push = [&](auto wrap) {
-            constexpr bool WRAP = decltype(wrap)::value;
-#pragma unroll
-            for (int l = 0; l < Q; ++l) {
-                constexpr bool NTK = (LB_NT_MASK & 16) ? true : NTSO;
-                if (l == REST) stf<P, NTK>(own(l) + kk, f[l], p.scale);
-                else if (MODE == MODE_PLAIN || !(m & (1u << l))) {
-                    if (CK[l] == 0) stf<P, NTK>(up(l) + kk, f[opp(l)], p.scale);
-                    else if (WRAP && wraps(l)) stf<P, NTSO>(upw(l), f[opp(l)], p.scale);
-                    else stf<P, NTSO>(up(l) + kk, f[opp(l)], p.scale);
-                } else stf<P, NTK>(own(l) + kk, f[opp(l)], p.scale);
-            }
-        };
-        if (anyw) push(std::true_type{});
-        else push(std::false_type{});
-    } else if constexpr (!ODD) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) stf<P, NTSE>(F + e + (int64_t)opp(l) * p.ps, f[l], p.scale);
-    } else {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) stf<P, NTSO>(F + odd_loc<true>(p, m, e, l, i, j, k), f[opp(l)], p.scale);
-    }
-}
-
-#ifndef LB_XCD_REMAP_CELLS
-#define LB_XCD_REMAP_CELLS (1u << 25)
-#endif
-// Tile of this workgroup. With p.xcd the tiles are dealt so that each of the
-// 8 XCDs (which receive workgroups round-robin) works through one contiguous
-// range of rows; the grid is then padded to a multiple of 8. It is set for
-// lattices of at least LB_XCD_REMAP_CELLS cells (an environment variable of
-// that name, read at lb_create, overrides it): the 768x384x384 porous
-// medium ran 5-7% faster with it, while the 256^3 cavity, the 512x128x128
-// duct and the 128x64x64 porous channel ran 1.5-3.8% faster without.
-LBD uint32_t tile_index(const StepParams& p) {
-    if (!p.xcd) return blockIdx.x;
-    const uint32_t per = gridDim.x >> 3;
-    return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-}
-
-// Mask bits that send a site to the special-site kernel. An even step reads
-// and writes only the site's own slots, so sites whose links cross a periodic
-// or symmetric face (M_BND) need no address resolution then and stay in the
-// bulk kernel; Zou-He sites need the face data on both parities.
-template <bool ODD>
-LBD constexpr uint32_t special_bits() { return ODD ? M_SPECIAL : M_ZH_ANY; }
-
-// Bulk step: every fluid site that is not special for this parity.
-template <int P, int RLX, bool ODD, bool DISS = false>
-LBD void step_bulk(const StepParams& p) {
-    const uint32_t tile = tile_index(p);
-    if (tile >= p.ntiles) return;
-    const uint32_t c = tile * p.tile + threadIdx.x;
-    // nq is a multiple of 64, so a wave is entirely inside or outside [0, nq)
-    const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
-    if (w >= p.nwaves) return;
-    if (LB_PLAIN_PATH && p.wclass[w] == 0) {
-        step_site<P, RLX, ODD, MODE_PLAIN, DISS>(p, c, M_FLUID);
-        return;
-    }
-    const uint32_t m = p.mask[c];
-    if ((m & (M_FLUID | special_bits<ODD>())) != M_FLUID) return;
-    step_site<P, RLX, ODD, MODE_BULK, DISS>(p, c, m);
-}
-
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(STEP_TILE_MAX) LB_OCC void k_step(StepParams p) {
-    step_bulk<P, RLX, ODD>(p);
-}
-
-// FP16 storage with TRT: the same kernel held to 64 VGPRs (8 waves per SIMD),
-// which it reaches without spills (69 VGPRs otherwise, 7 waves; ~4% slower on
-// the TRT cavity). BGK and the regularized scheme measured 9-10% slower with
-// the hint (128x64x64 porous channel, 512x128x128 duct) and keep k_step.
-template <int RLX, bool ODD>
-__global__ __launch_bounds__(STEP_TILE_MAX) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_h(StepParams p) {
-    step_bulk<LB_FP16_STORAGE, RLX, ODD>(p);
-}
-
-// Special sites, one thread per entry of the compact list.
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
-    const uint32_t t = blockIdx.x * TILE + threadIdx.x;
-    if (t >= (ODD ? p.nspecial : p.nzh)) return;
-    const uint32_t c = p.special[t];
-    step_site<P, RLX, ODD, MODE_SPECIAL>(p, c, p.mask[c]);
-}
-
-// Dissipation objective of the step about to run (diss_site): the step's
-// pre-collision part at every fluid site, nothing stored to f.
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(STEP_TILE_MAX) void k_diss(StepParams p) {
-    step_bulk<P, RLX, ODD, true>(p);
-}
-
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
-    const uint32_t t = blockIdx.x * TILE + threadIdx.x;
-    if (t >= (ODD ? p.nspecial : p.nzh)) return;
-    const uint32_t c = p.special[t];
-    step_site<P, RLX, ODD, MODE_SPECIAL, true>(p, c, p.mask[c]);
-}
-
-// ---------------------------------------------------------------------------
-// sparse layout: fluid sites only
-// ---------------------------------------------------------------------------
-// Compact site n (fluid sites in ascending dense order, so consecutive threads
-// mostly hold consecutive k of one lattice row) keeps population l at
-// f[l*ps + n]. An even step reads and writes only these slots. An odd step
-// reads link l from tab[link_slot(l)*ps + n], the element that the dense
-// resolver (odd_loc) names for it, mapped to compact indices at
-// lb_finalize: upstream slot, own slot for bounce-back, or the periodic /
-// mirrored source; it writes f*_opp(l) back there. FULL sites may carry
-// moving-wall or Zou-He bits, read from the dense mask; lattices with
-// neither run the FULL = false kernel, which loads no mask.
-//
-// The step reads the table in a compact form (k_sparse_pack): per wave of 64
-// compact sites and link, a 32-bit base location, and per site an offset
-// from it, SP_BOUNCE for the own slot l, or SP_FAR for a location out of the
-// offset's reach (boundary rows whose upstream row wraps, mirrored links),
-// which is then read from the 32-bit table.
-constexpr uint32_t SP_FAR = (1u << (8 * sizeof(SpCode))) - 1u, SP_BOUNCE = SP_FAR - 1u;
-
-// Odd-step locations of the 18 links of compact site n (loc[REST]: own slot).
-// Codes: an offset from the wave's base for the link (scalar loads), own slot
-// l (bounce-back), or the full 32-bit location. They are decoded with
-// selects, so that all 18 population loads after this can be in flight
-// together; the rare SP_FAR links are fixed up in one wave-uniform branch.
-LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
-    const uint32_t ps = (uint32_t)p.ps;
-    const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * (Q - 1);
-    uint32_t base[Q - 1], code[Q];
-#pragma unroll
-    for (int sl = 0; sl < Q - 1; ++sl) base[sl] = B[sl];
-    // used here: the loads are not sunk into a branch per link
-#pragma unroll
-    for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]));
-#pragma unroll
-    for (int l = 0; l < Q; ++l)
-        if (l != REST) code[l] = p.tabc[link_slot(l) * ps + n];
-    uint32_t far = 0;
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        if (l == REST) {
-            loc[l] = REST * ps + n;
-            continue;
-        }
-        loc[l] = code[l] == SP_BOUNCE ? (uint32_t)l * ps + n : base[link_slot(l)] + code[l];
-        far |= (code[l] == SP_FAR ? 1u : 0u) << link_slot(l);
-    }
-    if (__ballot(far != 0)) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l)
-            if (l != REST && ((far >> link_slot(l)) & 1u)) loc[l] = p.tab[link_slot(l) * ps + n];
-    }
-}
-
-template <int P, int RLX, bool ODD, bool FULL, bool DISS = false>
-LBD void step_sparse(const StepParams& p, const uint32_t n) {
-    using TS = typename Var<P>::TS;
-    using TC = typename Var<P>::TC;
-    constexpr bool SH = Var<P>::SH;
-    TS* F = (TS*)p.f;
-    const uint32_t ps = (uint32_t)p.ps;  // 19 * ps < 2^32 (lb_finalize)
-    const TC* A = (const TC*)p.acc;
-    const bool need_c = FULL || DISS || A || p.store_dv || p.rec;  // uniform
-    const uint32_t c = need_c ? p.site[n] : 0u;
-    const uint32_t m = FULL ? p.mask[c] : M_FLUID;
-    constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
-    constexpr bool NTLO = (LB_NT_MASK & 4) != 0, NTSO = (LB_NT_MASK & 8) != 0;
-
-    // population l of this site: a wave-uniform base plus the 32-bit lane
-    // offset n (n * sizeof(TS) < 2^32)
-    auto own = [&](int l) { return uniform_ptr(F + (int64_t)l * ps) + n; };
-    TC f[Q];
-    uint32_t loc[Q];
-    if constexpr (!ODD) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l), p.inv_scale);
-    } else {
-        sparse_locs(p, n, loc);
-#pragma unroll
-        for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLO>(l == REST ? own(l) : F + loc[l], p.inv_scale);
-    }
-    const TC rho0 = (TC)p.rho0;
-    if (FULL && p.moving && (m & M_MOVING)) ladd_apply<TC>(p, c, m, f);
-    if (FULL && (m & M_ZH_ANY)) {
-        int i, j, k;
-        decompose(c, p.d.ny, p.d.nzp, i, j, k);
-        zouhe_site<TC, SH>(p, m, i, j, k, rho0, f);
-    }
-    const int64_t nq = p.nq;
-    TC acc[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
-    if constexpr (DISS) {  // objective gather: nothing is stored to f
-        diss_site<TC, SH>(p, c, f, acc);
-        return;
-    }
-    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
-    TC vel[3];
-    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel);
-    if (p.store_dv) {
-        p.den[c] = (double)den;
-        p.vel[c] = (double)vel[0];
-        p.vel[nq + c] = (double)vel[1];
-        p.vel[2 * nq + c] = (double)vel[2];
-    } else if (FULL && p.moving && (m & M_MOVING)) {
-        p.den[c] = (double)den;
-    }
-    if (p.rec) {
-        TC* R = (TC*)p.rec;
-        R[c] = den;
-        R[nq + c] = vel[0];
-        R[2 * nq + c] = vel[1];
-        R[3 * nq + c] = vel[2];
-    }
-    if constexpr (!ODD) {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)), f[l], p.scale);
-    } else {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) stf<P, NTSO>(l == REST ? own(l) : F + loc[l], f[opp(l)], p.scale);
-    }
-}
-
-template <int P, int RLX, bool ODD, bool FULL>
-__global__ __launch_bounds__(STEP_TILE_MAX) void k_step_sp(StepParams p) {
-    const uint32_t tile = tile_index(p);
-    if (tile >= p.ntiles) return;
-    const uint32_t n = tile * p.tile + threadIdx.x;
-    if (n >= p.nf) return;
-    step_sparse<P, RLX, ODD, FULL>(p, n);
-}
-
-// the dissipation objective's gather in the sparse layout (see k_diss)
-template <int P, int RLX, bool ODD, bool FULL>
-__global__ __launch_bounds__(STEP_TILE_MAX) void k_diss_sp(StepParams p) {
-    const uint32_t tile = tile_index(p);
-    if (tile >= p.ntiles) return;
-    const uint32_t n = tile * p.tile + threadIdx.x;
-    if (n >= p.nf) return;
-    step_sparse<P, RLX, ODD, FULL, true>(p, n);
-}
-
-// ---------------------------------------------------------------------------
-// adjoint step
-// ---------------------------------------------------------------------------
-// The adjoint of an AA step is an AA step of the same parity on the cotangent
-// array: site x reads the cotangents of its post-collision populations from
-// the locations its forward step wrote them to, pulls them back through the
-// collision and the Zou-He faces (reverse order) and writes the cotangents of
-// its pre-collision populations to the locations its forward step read them
-// from. Those are the same 19 locations, so the step runs in place.
-struct AdjParams {
-    void* lam;          // cotangent array, population layout of f, compute type
-    const void* rec;    // (rho, u*) recorded by the forward recompute of this step
-    void* gacc;         // accumulated acceleration cotangent, 3 * nq, compute type
-    double* gface;      // accumulated face-velocity cotangent (face of gface_id), face layout x 3
-    int gface_id;
-    int seed;           // 1 on the objective's step
-    int obj_face, obj_axis;  // obj_face LB_OBJ_DISSIPATION: the dissipation objective
-    double obj_scale;   // 1 / (fluid sites on the objective face)
-    const void* dseed;  // dissipation objective: the seeds of diss_site
-};
-
-// Per-site core of the adjoint step: the collision's vector-Jacobian product
-// from the recorded (rho, u*), the objective's seed, the acceleration
-// cotangent and, on Zou-He sites (zh), the faces' VJP in reverse order.
-// lam: cotangents of the post-collision populations; mu: of the streamed ones.
-template <int P, int RLX>
-LBD void adj_core(const StepParams& p, const AdjParams& x, const uint32_t c, const uint32_t m, const bool zh, int i,
-                  int j, int k, const typename Var<P>::TC* lam, typename Var<P>::TC* mu) {
-    using TC = typename Var<P>::TC;
-    const int64_t nq = p.nq;
-    const TC* R = (const TC*)x.rec;
-    const TC rho = R[c];
-    const TC us[3] = {R[nq + c], R[2 * nq + c], R[3 * nq + c]};
-    TC acc[3];
-    const TC* A = (const TC*)p.acc;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) acc[a] = (TC)p.g[a] + (A ? A[(int64_t)a * nq + c] : TC(0));
-    TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
-    TC ga[3] = {TC(0), TC(0), TC(0)};
-    collide_vjp<RLX, TC>(lam, rho, us, acc, prm, mu, ga);
-    if (x.seed && x.obj_face == LB_OBJ_DISSIPATION) {
-        // dissipation objective: seeds written by diss_site in the forward pass
-        const TC* S = (const TC*)x.dseed;
-#pragma unroll
-        for (int l = 0; l < Q; ++l) mu[l] += S[(int64_t)l * nq + c];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ga[a] += S[(int64_t)(Q + a) * nq + c];
-    } else if (x.seed) {
-        // objective: mean over the face layer of rho u*_axis = j_axis + rho a_axis / 2
-        const int ax = x.obj_face >> 1;
-        const int pos = ax == 0 ? i : ax == 1 ? j : k;
-        const int n = ax == 0 ? p.d.nx : ax == 1 ? p.d.ny : p.d.nz;
-        if (pos == ((x.obj_face & 1) ? n - 1 : 0)) {
-            const TC sc = (TC)x.obj_scale;
-#pragma unroll
-            for (int l = 0; l < Q; ++l) mu[l] += sc * (TC(ccomp(l, x.obj_axis)) + TC(0.5) * acc[x.obj_axis]);
-            ga[x.obj_axis] += sc * TC(0.5) * rho;
-        }
-    }
-    if (x.gacc) {
-        TC* G = (TC*)x.gacc;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) G[(int64_t)a * nq + c] += ga[a];
-    }
-    if (zh) {
-        for (int fc = 5; fc >= 0; --fc) {
-            if (!(m & (M_ZH0 << fc))) continue;
-            int64_t fi;
-            if (fc < 2) fi = (int64_t)(j + 1) * p.NK + (k + 1);
-            else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
-            else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
-            const TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
-            const int kind = p.d.bc[fc];
-            const TC dbc = kind == FIXED_DEN ? (TC)p.fden[fc][fi] : rho;
-            TC nu[Q], gv[3] = {TC(0), TC(0), TC(0)};
-            zh_vjp<TC>(fc, kind, dbc, vb, mu, nu, gv);
-#pragma unroll
-            for (int l = 0; l < Q; ++l) mu[l] = nu[l];
-            if (x.gface && fc == x.gface_id)
-                for (int a = 0; a < 3; ++a) x.gface[3 * fi + a] += (double)gv[a];
-        }
-    }
-}
-
-// One site of the adjoint step. MODE_PLAIN / MODE_BULK address the
-// cotangents like the bulk step kernel (wave-uniform row bases, bounce-back
-// links from the link mask); MODE_SPECIAL resolves every link.
-template <int P, int RLX, bool ODD, int MODE>
-LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, const uint32_t m) {
-    using TC = typename Var<P>::TC;
-    constexpr bool SPECIAL = MODE == MODE_SPECIAL;
-    const int64_t nq = p.nq;
-    TC* L = (TC*)x.lam;
-    int i = 0, j = 0, k = 0;
-    if (SPECIAL || x.seed) decompose(c, p.d.ny, p.d.nzp, i, j, k);
-    const int64_t e = eidx(p, c);
-    const uint32_t nzp = (uint32_t)p.d.nzp;
-    const uint32_t row = SPECIAL ? 0u : __builtin_amdgcn_readfirstlane(c / nzp);
-    const uint32_t kk = c - row * nzp;
-    TC* const Lr = L + (int64_t)row * p.rs;
-    auto own = [&](int l) { return uniform_ptr(Lr + (int64_t)l * p.ps); };
-    auto up = [&](int l) {
-        return uniform_ptr(Lr - ((int64_t)CI[l] * p.d.ny + CJ[l]) * p.rs + (int64_t)opp(l) * p.ps - CK[l]);
-    };
-    // location of the cotangent of f_l's forward read (= of f*_opp(l)'s write)
-    const uint32_t nzm = (uint32_t)p.d.nz - 1u;
-    auto loc = [&](int l) -> TC* {
-        if constexpr (SPECIAL) return L + odd_loc<true>(p, m, e, l, i, j, k);
-        else if (l == REST || (MODE == MODE_BULK && (m & (1u << l)))) return own(l) + kk;
-        else if (MODE == MODE_BULK && (m & M_KWRAP) && ((CK[l] > 0 && kk == 0) || (CK[l] < 0 && kk == nzm)))
-            return up(l) + CK[l] + (CK[l] > 0 ? nzm : 0u);
-        else return up(l) + kk;
-    };
-    constexpr bool NTE = (LB_NT_MASK & 1) != 0;
-    TC lam[Q], mu[Q];
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        if constexpr (ODD) lam[opp(l)] = *loc(l);
-        else if constexpr (SPECIAL) lam[l] = L[e + (int64_t)opp(l) * p.ps];
-        else lam[l] = ld_stream<NTE>(own(opp(l)) + kk);
-    }
-    adj_core<P, RLX>(p, x, c, m, SPECIAL && (m & M_ZH_ANY), i, j, k, lam, mu);
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        if constexpr (ODD) *loc(l) = mu[l];
-        else if constexpr (SPECIAL) L[e + (int64_t)l * p.ps] = mu[l];
-        else st_stream<NTE>(own(l) + kk, mu[l]);
-    }
-}
-
-// Adjoint step over the non-special fluid sites (same tiling as k_step).
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
-    const uint32_t tile = tile_index(p);
-    if (tile >= p.ntiles) return;
-    const uint32_t c = tile * p.tile + threadIdx.x;
-    const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
-    if (w >= p.nwaves) return;
-    if (LB_PLAIN_PATH && p.wclass[w] == 0) {
-        adj_site<P, RLX, ODD, MODE_PLAIN>(p, x, c, M_FLUID);
-        return;
-    }
-    const uint32_t m = p.mask[c];
-    if ((m & (M_FLUID | special_bits<ODD>())) != M_FLUID) return;
-    adj_site<P, RLX, ODD, MODE_BULK>(p, x, c, m);
-}
-
-// Adjoint step over the special sites (compact list).
-template <int P, int RLX, bool ODD>
-__global__ __launch_bounds__(TILE) void k_adj_special(StepParams p, AdjParams x) {
-    const uint32_t t = blockIdx.x * TILE + threadIdx.x;
-    if (t >= (ODD ? p.nspecial : p.nzh)) return;
-    const uint32_t c = p.special[t];
-    adj_site<P, RLX, ODD, MODE_SPECIAL>(p, x, c, p.mask[c]);
-}
-
-// Adjoint step of the sparse layout, one thread per compact fluid site: the
-// cotangents sit where the forward step's populations do (own slots on even
-// steps, the decoded odd-step locations on odd steps). ZH: the lattice has
-// Zou-He faces (their VJP is compiled in).
-template <int P, int RLX, bool ODD, bool ZH>
-__global__ __launch_bounds__(TILE) void k_adj_sp(StepParams p, AdjParams x) {
-    using TC = typename Var<P>::TC;
-    const uint32_t tile = tile_index(p);
-    if (tile >= p.ntiles) return;
-    const uint32_t n = tile * p.tile + threadIdx.x;
-    if (n >= p.nf) return;
-    TC* L = (TC*)x.lam;
-    const uint32_t ps = (uint32_t)p.ps;
-    const uint32_t c = p.site[n];
-    const uint32_t m = ZH ? p.mask[c] : M_FLUID;
-    int i = 0, j = 0, k = 0;
-    if ((m & M_ZH_ANY) || x.seed) decompose(c, p.d.ny, p.d.nzp, i, j, k);
-    constexpr bool NTE = (LB_NT_MASK & 1) != 0;
-    uint32_t loc[Q];
-    TC lam[Q], mu[Q];
-    if constexpr (ODD) {
-        sparse_locs(p, n, loc);
-#pragma unroll
-        for (int l = 0; l < Q; ++l) lam[opp(l)] = L[loc[l]];
-    } else {
-#pragma unroll
-        for (int l = 0; l < Q; ++l) lam[l] = ld_stream<NTE>(uniform_ptr(L + (int64_t)opp(l) * ps) + n);
-    }
-    adj_core<P, RLX>(p, x, c, m, ZH && (m & M_ZH_ANY) != 0, i, j, k, lam, mu);
-#pragma unroll
-    for (int l = 0; l < Q; ++l) {
-        if constexpr (ODD) L[loc[l]] = mu[l];
-        else st_stream<NTE>(uniform_ptr(L + (int64_t)l * ps) + n, mu[l]);
-    }
-}
 
 // objective: sum over fluid sites of the face layer of rho u*_axis (FP64)
 __global__ __launch_bounds__(RED_BLOCK_ADJ) void k_obj(const uint32_t* mask, const double* den, const double* vel,
@@ -1013,6 +117,7 @@ __global__ void k_fill(T* p, size_t n, T v) {
     if (i < n) p[i] = v;
 }
 
+
 // ---------------------------------------------------------------------------
 // set-up kernels
 // ---------------------------------------------------------------------------
@@ -1064,117 +169,6 @@ __global__ void k_mask(const uint8_t* phase, const double* vel, uint32_t* mask, 
     if ((threadIdx.x & 63) == 0) wclass[c >> 6] = notplain ? 1 : 0;
 }
 
-// velocity of the site reached from x along +/-c_l for _comp_der_vel:
-// resolved through periodic / symmetric halos, zero for WALL/FIXED halos
-LBD void nb_vel(const Dims& d, const double* vel, uint32_t nq, int i, int j, int k, double* u) {
-    int l = 0, fl = 0;
-    u[0] = u[1] = u[2] = 0.0;
-    if (!resolve(d, i, j, k, l, fl)) return;
-    const int64_t y = ((int64_t)i * d.ny + j) * d.nzp + k;
-    for (int a = 0; a < 3; ++a) {
-        double v = vel[(int64_t)a * nq + y];
-        u[a] = (fl >> a & 1) ? -v : v;
-    }
-}
-
-// Solver._comp_der_vel, isothermal.pyx:958-1023
-__device__ void comp_der_vel(const Dims& d, const double* vel, uint32_t nq, uint32_t m, int i, int j, int k,
-                             int fdir, const double* u0, double* der) {
-    constexpr int FD[3][5] = {{14, 15, 16, 17, 18}, {4, 11, 12, 13, 18}, {3, 7, 10, 13, 17}};
-    der[0] = der[1] = der[2] = 0.0;
-    for (int fl = 0; fl < 5; ++fl) {
-        const int l = FD[fdir][fl];
-        double un[3], uo[3];
-        nb_vel(d, vel, nq, i + CI[l], j + CJ[l], k + CK[l], un);
-        nb_vel(d, vel, nq, i - CI[l], j - CJ[l], k - CK[l], uo);
-        const bool sn = m & (1u << opp(l));  // x + c_l is SOLID
-        const bool so = m & (1u << l);       // x - c_l is SOLID
-        double c1 = 0.5, c2 = 0.0, c3 = -0.5;
-        if (sn) {
-            if (so) {
-                c1 = 1.0;
-                c3 = -1.0;
-            } else {
-                c1 = 4.0 / 3.0;
-                c2 = -1.0;
-                c3 = -1.0 / 3.0;
-            }
-        } else if (so) {
-            c1 = 1.0 / 3.0;
-            c2 = 1.0;
-            c3 = -4.0 / 3.0;
-        }
-        const double w = nnz(l) == 1 ? 2.0 / 36.0 : 1.0 / 36.0;
-        const double c0 = 2.0 * w * 3.0;
-        der[0] += c0 * (c1 * un[0] + c2 * u0[0] + c3 * uo[0]);
-        der[1] += c0 * (c1 * un[1] + c2 * u0[1] + c3 * uo[1]);
-        der[2] += c0 * (c1 * un[2] + c2 * u0[2] + c3 * uo[2]);
-    }
-}
-
-// Solver._init_f, isothermal.pyx:1028-1105: f1 = P(heq + (1-omega)(-2 tau/3) rho S),
-// written as the state after an even AA step (f*_l in slot opp(l)).
-template <int P>
-__global__ void k_init(StepParams p) {
-    using TS = typename Var<P>::TS;
-    constexpr bool SH = Var<P>::SH;
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= p.nq) return;
-    const uint32_t m = p.mask[c];
-    if (!(m & M_FLUID)) return;
-    int i, j, k;
-    decompose(c, p.d.ny, p.d.nzp, i, j, k);
-    const int64_t nq = p.nq;
-    const double inv_tau = p.prm[0], tau = 1.0 / inv_tau, rlx_cff = 1.0 - inv_tau;
-    const double den = p.den[c];
-    double u[3] = {p.vel[c], p.vel[nq + c], p.vel[2 * nq + c]};
-    double h[10], dx[3], dy[3], dz[3], f[Q];
-    heq<double>(den, u, h);
-    comp_der_vel(p.d, p.vel, p.nq, m, i, j, k, 0, u, dx);
-    comp_der_vel(p.d, p.vel, p.nq, m, i, j, k, 1, u, dy);
-    comp_der_vel(p.d, p.vel, p.nq, m, i, j, k, 2, u, dz);
-    const double m2 = -2.0 * tau * (1.0 / 3.0) * den;
-    h[4] += rlx_cff * m2 * dx[0];
-    h[5] += rlx_cff * m2 * 0.5 * (dx[1] + dy[0]);
-    h[6] += rlx_cff * m2 * 0.5 * (dx[2] + dz[0]);
-    h[7] += rlx_cff * m2 * dy[1];
-    h[8] += rlx_cff * m2 * 0.5 * (dy[2] + dz[1]);
-    h[9] += rlx_cff * m2 * dz[2];
-    if (SH) h[0] = den - p.rho0;
-    project<double, false>(h, f);
-    TS* F = (TS*)p.f;
-    const int64_t e = eidx(p, c);
-    for (int l = 0; l < Q; ++l) stf<P, false>(F + e + (int64_t)opp(l) * p.ps, (typename Var<P>::TC)f[l], p.scale);
-}
-
-// Post-collision populations of the newest step, unpacked into the reference's
-// f1 layout restricted to the interior: out[l][i][j][k] (f64).
-template <int P>
-__global__ void k_post_f(StepParams p, int next_odd, double* out) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= p.nq) return;
-    int i, j, k;
-    decompose(c, p.d.ny, p.d.nzp, i, j, k);
-    if (k >= p.d.nz) return;
-    const int64_t nc = (int64_t)p.d.nx * p.d.ny * p.d.nz;
-    const int64_t o = ((int64_t)i * p.d.ny + j) * p.d.nz + k;
-    const uint32_t m = p.mask[c];
-    const typename Var<P>::TS* F = (const typename Var<P>::TS*)p.f;
-    for (int l = 0; l < Q; ++l) {
-        double v = 0.0;
-        if (m & M_FLUID) {
-            const int64_t e = eidx(p, c);
-            const int lo = opp(l);
-            int64_t loc;
-            if (next_odd) loc = e + (int64_t)lo * p.ps;
-            else if (!p.tab) loc = odd_loc(p, m, e, lo, i, j, k);
-            else loc = lo == REST ? e + (int64_t)REST * p.ps : p.tab[(int64_t)link_slot(lo) * p.ps + e];
-            v = (double)ldf<P, false>(F + loc, p.inv_scale);
-            if (Var<P>::SH) v += (l == REST ? 12.0 / 36.0 : nnz(l) == 1 ? 2.0 / 36.0 : 1.0 / 36.0) * p.rho0;
-        }
-        out[(int64_t)l * nc + o] = v;
-    }
-}
 
 // interior contiguous (den[nc], vel[nc][3]) <-> padded SoA device fields
 __global__ void k_import_dv(const double* st, double* den, double* vel, Dims d, uint32_t nq, int with_den) {
@@ -1324,6 +318,281 @@ __global__ __launch_bounds__(RED_BLOCK) void k_conv1(const uint32_t* mask, const
 }
 
 // ---------------------------------------------------------------------------
+// calc_flow_info (include/solver/solver.cpp:64-129): FP64 sums, extrema and
+// the fluid-site count in one pass. Slots FI_DEN..FI_CNT are sums, FI_MAXDEN
+// and FI_MAXU maxima, FI_MINDEN a minimum; the extrema of the density start
+// from the reference density and the speed maximum from 0, as in the
+// reference. The sites come from an accessor: a solver's padded SoA fields
+// (SolverSites) or caller-provided interior C-order fields (PlainSites).
+// ---------------------------------------------------------------------------
+enum {
+    FI_DEN, FI_UX, FI_UY, FI_UZ, FI_U2, FI_JX, FI_JY, FI_JZ, FI_J2, FI_CNT,  // sums
+    FI_MAXDEN, FI_MINDEN, FI_MAXU, FLOW_N
+};
+constexpr int FLOW_SUMS = FI_CNT + 1;
+
+LBD double flow_op(int q, double a, double b) {
+    return q < FLOW_SUMS ? a + b : q == FI_MINDEN ? fmin(a, b) : fmax(a, b);
+}
+
+struct SolverSites {
+    const uint32_t* mask;
+    const double* den;
+    const double* vel;  // 3 * nq, SoA
+    uint32_t n;
+    LBD bool get(uint32_t c, double& r, double& u, double& v, double& w) const {
+        if (!(mask[c] & M_FLUID)) return false;
+        r = den[c];
+        u = vel[c];
+        v = vel[n + c];
+        w = vel[2 * (int64_t)n + c];
+        return true;
+    }
+};
+
+struct PlainSites {
+    const uint8_t* phase;  // [n], SOLID = 0
+    const double* den;     // [n]
+    const double* vel;     // [n][3]
+    uint32_t n;
+    LBD bool get(uint32_t c, double& r, double& u, double& v, double& w) const {
+        if (phase[c] == SOLID) return false;
+        r = den[c];
+        u = vel[3 * (int64_t)c];
+        v = vel[3 * (int64_t)c + 1];
+        w = vel[3 * (int64_t)c + 2];
+        return true;
+    }
+};
+
+template <class S>
+__global__ __launch_bounds__(RED_BLOCK) void k_flow1(S sites, double rho0, double* part) {
+    double a[FLOW_N];
+#pragma unroll
+    for (int q = 0; q < FLOW_N; ++q) a[q] = q < FLOW_SUMS ? 0.0 : q == FI_MAXU ? 0.0 : rho0;
+    for (uint32_t c = blockIdx.x * RED_BLOCK + threadIdx.x; c < sites.n; c += gridDim.x * RED_BLOCK) {
+        double r, u, v, w;
+        if (!sites.get(c, r, u, v, w)) continue;
+        const double u2 = sqrt(u * u + v * v + w * w);
+        a[FI_DEN] += r;
+        a[FI_UX] += u;
+        a[FI_UY] += v;
+        a[FI_UZ] += w;
+        a[FI_U2] += u2;
+        a[FI_JX] += r * u;
+        a[FI_JY] += r * v;
+        a[FI_JZ] += r * w;
+        a[FI_J2] += r * u2;
+        a[FI_CNT] += 1.0;
+        a[FI_MAXDEN] = fmax(a[FI_MAXDEN], r);
+        a[FI_MINDEN] = fmin(a[FI_MINDEN], r);
+        a[FI_MAXU] = fmax(a[FI_MAXU], u2);
+    }
+    __shared__ double sh[RED_BLOCK / 64][FLOW_N];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < FLOW_N; ++q) {
+        double s = a[q];
+        for (int o = 32; o > 0; o >>= 1) s = flow_op(q, s, __shfl_xor(s, o, 64));
+        if (lane == 0) sh[wv][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < FLOW_N) {
+        const int q = threadIdx.x;
+        double s = sh[0][q];
+        for (int w2 = 1; w2 < RED_BLOCK / 64; ++w2) s = flow_op(q, s, sh[w2][q]);
+        part[(int64_t)blockIdx.x * FLOW_N + q] = s;
+    }
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_flow2(const double* part, int nparts, double rho0, double* out) {
+    double a[FLOW_N];
+#pragma unroll
+    for (int q = 0; q < FLOW_N; ++q) a[q] = q < FLOW_SUMS ? 0.0 : q == FI_MAXU ? 0.0 : rho0;
+    for (int b = threadIdx.x; b < nparts; b += RED_BLOCK)
+#pragma unroll
+        for (int q = 0; q < FLOW_N; ++q) a[q] = flow_op(q, a[q], part[(int64_t)b * FLOW_N + q]);
+    __shared__ double sh[RED_BLOCK / 64][FLOW_N];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < FLOW_N; ++q) {
+        double s = a[q];
+        for (int o = 32; o > 0; o >>= 1) s = flow_op(q, s, __shfl_xor(s, o, 64));
+        if (lane == 0) sh[wv][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < FLOW_N) {
+        const int q = threadIdx.x;
+        double s = sh[0][q];
+        for (int w2 = 1; w2 < RED_BLOCK / 64; ++w2) s = flow_op(q, s, sh[w2][q]);
+        out[q] = s;
+    }
+}
+
+// IsothermalFlowInfo from the reduced slots (solver.cpp:112-128)
+static void flow_info_from(const double* r, lb_flow_info_t* info) {
+    info->tot_jx = r[FI_JX];
+    info->tot_jy = r[FI_JY];
+    info->tot_jz = r[FI_JZ];
+    info->tot_j2 = r[FI_J2];
+    info->tot_den = r[FI_DEN];
+    const double n = r[FI_CNT];
+    const double ave_cff = n > 0 ? 1.0 / n : 0.0;
+    info->ave_ux = ave_cff * r[FI_UX];
+    info->ave_uy = ave_cff * r[FI_UY];
+    info->ave_uz = ave_cff * r[FI_UZ];
+    info->ave_u2 = ave_cff * r[FI_U2];
+    info->max_u2 = r[FI_MAXU];
+    info->max_den = r[FI_MAXDEN];
+    info->min_den = r[FI_MINDEN];
+    info->fluid_sites = n;
+}
+
+// ---------------------------------------------------------------------------
+// the Solver's halo-padded host fields (lb_bind_host_fields / lb_evolve_host),
+// staged on the device in the host layout: phase u8 [NI][NJ][NK], den f64
+// [NI][NJ][NK], vel and acc f64 [NI][NJ][NK][3] (isothermal.pyx:243-248)
+// ---------------------------------------------------------------------------
+LBD int64_t host_idx(const Dims& d, int i, int j, int k) {
+    return ((int64_t)(i + 1) * (d.ny + 2) + (j + 1)) * (d.nz + 2) + (k + 1);
+}
+
+constexpr unsigned SYNC_PHASE = 1u, SYNC_WALLS = 2u, SYNC_ACC = 4u;
+
+// Inputs of an evolve call from the staged host fields: den/vel of every
+// interior site (the wall velocities and the densities that Ladd's term reads
+// in the first step, isothermal.pyx:1117-1119), the acceleration field in the
+// compute type (acc may be null: only the flag is raised). Flags, OR-ed per
+// wave into *flags: SYNC_PHASE if a site's fluid/solid state differs from the
+// finalized phase, SYNC_WALLS if a SOLID site's velocity changed, SYNC_ACC if
+// the acceleration field is nonzero anywhere.
+template <class TC>
+__global__ void k_sync_in(const uint8_t* hphase, const double* hden, const double* hvel, const double* hacc,
+                          const uint8_t* phase, double* den, double* vel, TC* acc, Dims d, uint32_t nq,
+                          unsigned int* flags) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned fl = 0;
+    if (c < nq) {
+        int i, j, k;
+        decompose(c, d.ny, d.nzp, i, j, k);
+        if (k < d.nz) {
+            const int64_t o = host_idx(d, i, j, k);
+            const bool fluid = phase[c] != SOLID;
+            if ((hphase[o] != SOLID) != fluid) fl |= SYNC_PHASE;
+            const double v0 = hvel[3 * o], v1 = hvel[3 * o + 1], v2 = hvel[3 * o + 2];
+            if (!fluid && (v0 != vel[c] || v1 != vel[nq + c] || v2 != vel[2 * (int64_t)nq + c])) fl |= SYNC_WALLS;
+            den[c] = hden[o];
+            vel[c] = v0;
+            vel[nq + c] = v1;
+            vel[2 * (int64_t)nq + c] = v2;
+            const double a0 = hacc[3 * o], a1 = hacc[3 * o + 1], a2 = hacc[3 * o + 2];
+            if (a0 != 0.0 || a1 != 0.0 || a2 != 0.0) fl |= SYNC_ACC;
+            if (acc) {
+                acc[c] = (TC)a0;
+                acc[nq + c] = (TC)a1;
+                acc[2 * (int64_t)nq + c] = (TC)a2;
+            }
+        } else if (acc) {
+            acc[c] = acc[nq + c] = acc[2 * (int64_t)nq + c] = TC(0);
+        }
+    }
+    const unsigned w = (__ballot((fl & SYNC_PHASE) != 0) ? SYNC_PHASE : 0u) |
+                       (__ballot((fl & SYNC_WALLS) != 0) ? SYNC_WALLS : 0u) |
+                       (__ballot((fl & SYNC_ACC) != 0) ? SYNC_ACC : 0u);
+    if ((threadIdx.x & 63) == 0 && w) atomicOr(flags, w);
+}
+
+// den/vel of the fluid sites into the staged host fields (the SOLID sites
+// keep the host's values: the reference never writes them)
+__global__ void k_scatter_fluid(const uint32_t* mask, const double* den, const double* vel, double* hden,
+                                double* hvel, Dims d, uint32_t nq) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nq || !(mask[c] & M_FLUID)) return;
+    int i, j, k;
+    decompose(c, d.ny, d.nzp, i, j, k);
+    const int64_t o = host_idx(d, i, j, k);
+    hden[o] = den[c];
+    hvel[3 * o] = vel[c];
+    hvel[3 * o + 1] = vel[nq + c];
+    hvel[3 * o + 2] = vel[2 * (int64_t)nq + c];
+}
+
+// _enforce_den_vel_per_sym_bc (isothermal.pyx:582-686) on the staged host
+// fields, one axis per launch in the reference's order x, y, z: the x pass
+// covers interior j,k, the y pass all i and interior k, the z pass all i,j.
+// A periodic face copies the opposite interior layer into its halo, a
+// symmetric face mirrors the adjacent layer with the normal velocity negated.
+__global__ void k_halo_dv(double* hden, double* hvel, Dims d, int axis) {
+    const int NI = d.nx + 2, NJ = d.ny + 2, NK = d.nz + 2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int a, b;  // the two coordinates across the face
+    int n1, n2;
+    if (axis == 0) { n1 = NJ - 2; n2 = NK - 2; }
+    else if (axis == 1) { n1 = NI; n2 = NK - 2; }
+    else { n1 = NI; n2 = NJ; }
+    if (t >= (int64_t)n1 * n2) return;
+    a = (int)(t / n2);
+    b = (int)(t - (int64_t)a * n2);
+    const int N = axis == 0 ? NI : axis == 1 ? NJ : NK;
+    auto at = [&](int pos) -> int64_t {
+        if (axis == 0) return ((int64_t)pos * NJ + (a + 1)) * NK + (b + 1);
+        if (axis == 1) return ((int64_t)a * NJ + pos) * NK + (b + 1);
+        return ((int64_t)a * NJ + b) * NK + pos;
+    };
+    auto copy = [&](int dst, int src, bool neg) {
+        const int64_t o = at(dst), s = at(src);
+        hden[o] = hden[s];
+        for (int q = 0; q < 3; ++q) {
+            const double v = hvel[3 * s + q];
+            hvel[3 * o + q] = (neg && q == axis) ? -v : v;
+        }
+    };
+    const int lo = d.bc[2 * axis], hi = d.bc[2 * axis + 1];
+    if (lo == PERIODIC) copy(N - 1, 1, false);
+    else if (lo == SYMMETRIC) copy(0, 1, true);
+    if (hi == PERIODIC) copy(0, N - 2, false);
+    else if (hi == SYMMETRIC) copy(N - 1, N - 2, true);
+}
+
+// ---------------------------------------------------------------------------
+// FP16 storage guard: the largest stored magnitude (|bits| & 0x7fff, so that
+// inf and NaN, exponent all ones, compare above every finite value) and an
+// exact rescale by a power of two
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_f16_amax(const uint4* f, size_t n8, unsigned int* out) {
+    uint32_t m = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n8; i += (size_t)gridDim.x * 256) {
+        const uint4 v = f[i];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m = max(m, max(w[q] & 0x7fffu, (w[q] >> 16) & 0x7fffu));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ __launch_bounds__(256) void k_f16_scale(_Float16* f, size_t n, float factor) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        f[i] = (_Float16)((float)f[i] * factor);
+}
+
+// bound of |f - w rho0| implied by the Zou-He data of one face:
+// |den - rho0| + 3 |u|_1 + 4.5 |u|_1^2 (FIXED_DEN faces use den, FIXED_VEL
+// faces the velocity; both are bounded)
+__global__ void k_face_bound(const double* fden, const double* fvel, size_t n, double rho0, double* part) {
+    double a = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double u = fabs(fvel[3 * i]) + fabs(fvel[3 * i + 1]) + fabs(fvel[3 * i + 2]);
+        a = fmax(a, fabs(fden[i] - rho0) + 3.0 * u + 4.5 * u * u);
+    }
+    a = wave_max(a);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
 struct lb_solver {
@@ -1356,6 +625,7 @@ struct lb_solver {
     // sparse layout: compact index per site, site per compact index, odd-step
     // table; nfp = fluid sites rounded up to 64 (the population stride)
     bool sparse = false;
+    int layout = LB_LAYOUT_AUTO;  // resolved at lb_create, or for LB_LAYOUT_AUTO at the first lb_finalize
     uint32_t* cidx = nullptr;
     uint32_t* site = nullptr;
     uint32_t* tab = nullptr;
@@ -1366,11 +636,23 @@ struct lb_solver {
     double* den = nullptr;
     double* vel = nullptr;
     void* acc = nullptr;
+    bool acc_on = false;  // the step kernels read acc (it is nonzero somewhere)
     double* fden[6] = {};
     double* fvel[6] = {};
     double* stage = nullptr;  // 4 * ncell doubles (import / export)
-    double* part = nullptr;   // reduction partials
-    double* red = nullptr;    // 8 doubles
+    double* part = nullptr;   // reduction partials (RED_PARTS x FLOW_N)
+    double* red = nullptr;    // FLOW_N doubles
+    // the Solver's halo-padded host fields (lb_bind_host_fields) and their
+    // staged copies on the device, ncnt = NI * NJ * NK sites
+    uint8_t* h_phase = nullptr;
+    double *h_den = nullptr, *h_vel = nullptr, *h_acc = nullptr;
+    uint8_t* st_phase = nullptr;
+    double *st_den = nullptr, *st_vel = nullptr, *st_acc = nullptr;
+    int64_t ncnt = 0;
+    // FP16 storage guard: steps between checks of the stored magnitudes, steps
+    // since the last check, and whether the scale was chosen automatically
+    uint32_t f16_check = 64, f16_since = 0;
+    bool f16_auto = true;
     double* uprev = nullptr;
     bool have_uprev = false;
     bool phase_set = false;
@@ -1432,7 +714,7 @@ static StepParams params(const lb_solver* s) {
     p.mask = s->mask;
     p.den = s->den;
     p.vel = s->vel;
-    p.acc = s->acc;
+    p.acc = s->acc_on ? s->acc : nullptr;
     for (int fc = 0; fc < 6; ++fc) {
         p.fden[fc] = s->fden[fc];
         p.fvel[fc] = s->fvel[fc];
@@ -1474,16 +756,33 @@ static StepParams params(const lb_solver* s) {
     return p;
 }
 
+// An input of the step changed (or the state moved on): the forward pass's
+// record no longer matches, and an adjoint pass recorded before refers to
+// other inputs, so the reverse pass is refused until the next lb_adj_forward.
+static void inputs_changed(lb_solver* s) {
+    s->rec_fwd = false;
+    s->adj_ready = false;
+}
+
+static int face_bound(lb_solver* s, double* out);
+
+// LB_LAYOUT_AUTO picks the sparse layout from this fraction of solid cells on
+// (LB_AUTO_SPARSE_SOLID in the environment overrides it)
+#ifndef LB_AUTO_SPARSE_SOLID
+#define LB_AUTO_SPARSE_SOLID 0.05
+#endif
+
 static unsigned blocks_for(uint32_t n, unsigned b) { return (n + b - 1) / b; }
 
 // _check_bc, isothermal.pyx:378-426
 static int check_config(const lb_config* c) {
     if (c->lat[0] < 1 || c->lat[1] < 1 || c->lat[2] < 1) return fail(LB_EINVAL, "Invalid lattice size input.");
-    // sites are indexed with 32 bits: nx * ny * round_up(nz, 64) < 2^32
+    // sites are indexed with 32-bit unsigned integers, and the site counts
+    // of the launch arithmetic are ints: nx * ny * round_up(nz, 64) < 2^31
     const uint64_t padded = (uint64_t)c->lat[0] * (uint64_t)c->lat[1] * (((uint64_t)c->lat[2] + 63) / 64 * 64);
-    if (padded >= (1ull << 32))
-        return fail(LB_EINVAL, "lattice of %llu cells (z padded to a multiple of 64) exceeds the engine's 2^32 limit",
-                    (unsigned long long)padded);
+    if (padded >= (1ull << 31))
+        return fail(LB_EINVAL, "lattice of %llu cells (z padded to a multiple of 64) exceeds the engine's limit "
+                    "of 2^31 padded cells", (unsigned long long)padded);
     if (c->rlx_scheme < 0 || c->rlx_scheme > 2) return fail(LB_EINVAL, "Invalid relaxation scheme input.");
     if (!(c->kvisc > 0.0)) return fail(LB_EINVAL, "Invalid kinematic viscosity input.");
     for (int f = 0; f < 6; ++f)
@@ -1502,10 +801,7 @@ static int check_config(const lb_config* c) {
     if ((fx[0] && (fx[1] || fx[2])) || (fx[1] && (fx[0] || fx[2])) || (fx[2] && (fx[0] || fx[1])))
         return fail(LB_EINVAL, "Invalid fixed variable bc setup.");
     if (c->precision < LB_FP64 || c->precision > LB_FP32_STORAGE) return fail(LB_EINVAL, "Invalid precision.");
-    if (c->layout < LB_LAYOUT_ROWS || c->layout > LB_LAYOUT_SPARSE) return fail(LB_EINVAL, "Invalid layout.");
-    const int64_t nzp = (c->lat[2] + 63) / 64 * 64;
-    const int64_t nq = (int64_t)c->lat[0] * c->lat[1] * nzp;
-    if (nq >= (int64_t)1 << 31) return fail(LB_EINVAL, "Lattice too large (%lld padded sites).", (long long)nq);
+    if (c->layout < LB_LAYOUT_ROWS || c->layout > LB_LAYOUT_AUTO) return fail(LB_EINVAL, "Invalid layout.");
     return LB_OK;
 }
 
@@ -1524,6 +820,7 @@ static int import_acc_from_stage(lb_solver* s) {
     else
         k_import_acc<float><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->stage, (float*)s->acc, s->d, s->nq);
     HIPCHK(hipGetLastError());
+    s->acc_on = true;
     return LB_OK;
 }
 
@@ -1752,46 +1049,6 @@ __global__ void k_maxdev(const uint32_t* mask, const double* den, const double* 
     if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-template <int P>
-static void launch_init(lb_solver* s) {
-    StepParams p = params(s);
-    k_init<P><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p);
-}
-
-template <int P, int RLX>
-static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
-    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
-    if (p.tab) {  // sparse layout: one kernel, with the mask only where needed
-        if (p.moving || p.nzh) {
-            if (odd) k_step_sp<P, RLX, true, true><<<grid, p.tile, p.lds[1], st>>>(p);
-            else k_step_sp<P, RLX, false, true><<<grid, p.tile, p.lds[0], st>>>(p);
-        } else {
-            if (odd) k_step_sp<P, RLX, true, false><<<grid, p.tile, p.lds[1], st>>>(p);
-            else k_step_sp<P, RLX, false, false><<<grid, p.tile, p.lds[0], st>>>(p);
-        }
-        return;
-    }
-    if constexpr (P == LB_FP16_STORAGE && RLX == TRT) {
-        if (odd) k_step_h<RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
-        else k_step_h<RLX, false><<<grid, p.tile, p.lds[0], st>>>(p);
-    } else {
-        if (odd) k_step<P, RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
-        else k_step<P, RLX, false><<<grid, p.tile, p.lds[0], st>>>(p);
-    }
-    const uint32_t ns = odd ? p.nspecial : p.nzh;
-    if (ns) {
-        const unsigned g2 = (ns + TILE - 1) / TILE;
-        if (odd) k_step_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
-        else k_step_special<P, RLX, false><<<g2, TILE, 0, st>>>(p);
-    }
-}
-
-template <int P>
-static void launch_step_rlx(const StepParams& p, int rlx, bool odd, hipStream_t st) {
-    if (rlx == BGK) launch_step<P, BGK>(p, odd, st);
-    else if (rlx == TRT) launch_step<P, TRT>(p, odd, st);
-    else launch_step<P, REGUL>(p, odd, st);
-}
 
 static void launch_any_step(const lb_solver* s, const StepParams& p, bool odd) {
     switch (s->prec) {
@@ -1800,70 +1057,6 @@ static void launch_any_step(const lb_solver* s, const StepParams& p, bool odd) {
     case LB_FP32_STORAGE: launch_step_rlx<LB_FP32_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
     default: launch_step_rlx<LB_FP16_STORAGE>(p, s->cfg.rlx_scheme, odd, s->stream); break;
     }
-}
-
-template <int P, int RLX>
-static void launch_adj(const lb_solver* s, const StepParams& ps, const AdjParams& x, bool odd) {
-    StepParams p = ps;  // the adjoint kernels run TILE threads per workgroup
-    p.tile = TILE;
-    p.ntiles = ((p.tab ? (uint32_t)p.ps : p.nq) + TILE - 1) / TILE;
-    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
-    if (p.tab) {  // sparse layout
-        if (p.nzh) {
-            if (odd) k_adj_sp<P, RLX, true, true><<<grid, TILE, 0, s->stream>>>(p, x);
-            else k_adj_sp<P, RLX, false, true><<<grid, TILE, 0, s->stream>>>(p, x);
-        } else {
-            if (odd) k_adj_sp<P, RLX, true, false><<<grid, TILE, 0, s->stream>>>(p, x);
-            else k_adj_sp<P, RLX, false, false><<<grid, TILE, 0, s->stream>>>(p, x);
-        }
-        return;
-    }
-    if (odd) k_adj<P, RLX, true><<<grid, TILE, 0, s->stream>>>(p, x);
-    else k_adj<P, RLX, false><<<grid, TILE, 0, s->stream>>>(p, x);
-    const uint32_t ns = odd ? p.nspecial : p.nzh;
-    if (ns) {
-        const unsigned g2 = (ns + TILE - 1) / TILE;
-        if (odd) k_adj_special<P, RLX, true><<<g2, TILE, 0, s->stream>>>(p, x);
-        else k_adj_special<P, RLX, false><<<g2, TILE, 0, s->stream>>>(p, x);
-    }
-}
-
-template <int P>
-static void launch_adj_rlx(const lb_solver* s, const StepParams& p, const AdjParams& x, bool odd) {
-    if (s->cfg.rlx_scheme == BGK) launch_adj<P, BGK>(s, p, x, odd);
-    else if (s->cfg.rlx_scheme == TRT) launch_adj<P, TRT>(s, p, x, odd);
-    else launch_adj<P, REGUL>(s, p, x, odd);
-}
-
-// the dissipation objective's gather before the step of parity `odd`
-template <int P, int RLX>
-static void launch_diss(const StepParams& p, bool odd, hipStream_t st) {
-    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
-    if (p.tab) {  // sparse layout
-        if (p.moving || p.nzh) {
-            if (odd) k_diss_sp<P, RLX, true, true><<<grid, p.tile, 0, st>>>(p);
-            else k_diss_sp<P, RLX, false, true><<<grid, p.tile, 0, st>>>(p);
-        } else {
-            if (odd) k_diss_sp<P, RLX, true, false><<<grid, p.tile, 0, st>>>(p);
-            else k_diss_sp<P, RLX, false, false><<<grid, p.tile, 0, st>>>(p);
-        }
-        return;
-    }
-    if (odd) k_diss<P, RLX, true><<<grid, p.tile, 0, st>>>(p);
-    else k_diss<P, RLX, false><<<grid, p.tile, 0, st>>>(p);
-    const uint32_t ns = odd ? p.nspecial : p.nzh;
-    if (ns) {
-        const unsigned g2 = (ns + TILE - 1) / TILE;
-        if (odd) k_diss_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
-        else k_diss_special<P, RLX, false><<<g2, TILE, 0, st>>>(p);
-    }
-}
-
-template <int P>
-static void launch_diss_rlx(const lb_solver* s, const StepParams& p, bool odd) {
-    if (s->cfg.rlx_scheme == BGK) launch_diss<P, BGK>(p, odd, s->stream);
-    else if (s->cfg.rlx_scheme == TRT) launch_diss<P, TRT>(p, odd, s->stream);
-    else launch_diss<P, REGUL>(p, odd, s->stream);
 }
 
 // sum of one nq-long row of the compute type, 256 partial sums into part
@@ -1891,6 +1084,66 @@ int lb_device_count(void) {
     return n;
 }
 
+// Launch defaults of a population layout, then the environment overrides.
+static void layout_defaults(lb_solver* s, int layout) {
+    const bool sparse = layout == LB_LAYOUT_SPARSE;
+    // The sparse FP32-storage kernels measured 5% faster on the 768x384x384
+    // medium in the plain round-robin order (the sparse FP64 kernel 4% faster
+    // with the XCD order, FP16 the same either way).
+    if (sparse && (s->prec == LB_FP32_SHIFTED || s->prec == LB_FP32_STORAGE)) s->xcd_cells = UINT64_MAX;
+    if (const char* env = getenv("LB_XCD_REMAP_CELLS")) {
+        s->xcd_cells = strtoull(env, nullptr, 10);
+        s->tune_xcd = false;
+    }
+    // Unused dynamic LDS caps the bulk step's occupancy. The FP32 kernel needs
+    // few registers (8 waves per SIMD); its even step streamed 3-4% faster at
+    // 6 waves per SIMD (26 KB per workgroup of 256 threads), while capping the
+    // other variants or the odd step measured slower.
+    if (s->prec == LB_FP32_SHIFTED && !sparse) s->step_lds[0] = 26000;
+    if (const char* env = getenv("LB_STEP_LDS_EVEN")) s->step_lds[0] = (uint32_t)strtoul(env, nullptr, 10);
+    if (const char* env = getenv("LB_STEP_LDS_ODD")) s->step_lds[1] = (uint32_t)strtoul(env, nullptr, 10);
+    for (uint32_t& b : s->step_lds) b = b > 65536u ? 65536u : b;  // 64 KiB keeps one workgroup per CU
+    // Bulk step workgroups: the dense FP64 kernel (4 waves per SIMD) measured
+    // 4% faster with 512 threads on the 256^3 cavity (1.3% slower on the
+    // 512x128x128 duct, even on porous media); FP32 storage measured 8% slower
+    // with 512 on the cavity, and the sparse FP64 odd step holds 3 waves per
+    // SIMD, of which a 512-thread workgroup would leave 2: those and the
+    // rest keep 256. LB_STEP_TILE (a multiple of 64 up to 512) overrides it.
+    if (s->prec == LB_FP64 && !sparse) s->step_tile = 512;
+    if (const char* env = getenv("LB_STEP_TILE")) {
+        const unsigned long t = strtoul(env, nullptr, 10);
+        if (t >= 64 && t <= (unsigned long)STEP_TILE_MAX && t % 64 == 0) s->step_tile = (uint32_t)t;
+    }
+}
+
+// Strides and population array of a layout: row-interleaved f[i][j][l][k]
+// (ROWS), plain SoA f[l][i][j][k], or SoA with the population stride padded
+// by 2304 bytes so that equal offsets of different populations do not share
+// a channel. The sparse layout sizes its arrays from the fluid sites at
+// lb_finalize (build_sparse).
+static int layout_arrays(lb_solver* s, int layout) {
+    s->layout = layout;
+    s->sparse = layout == LB_LAYOUT_SPARSE;
+    if (s->sparse) {
+        s->fsize = 0;
+        return LB_OK;
+    }
+    const int64_t rows = (int64_t)s->d.nx * s->d.ny;
+    if (layout == LB_LAYOUT_SOA || layout == LB_LAYOUT_SOA_PADDED) {
+        s->rs = s->d.nzp;
+        s->ps = (int64_t)s->nq + (layout == LB_LAYOUT_SOA_PADDED ? (int64_t)(2304 / ts_size(s->prec)) : 0);
+        s->fsize = (int64_t)Q * s->ps;
+    } else {
+        s->rs = (int64_t)Q * s->d.nzp;
+        s->ps = s->d.nzp;
+        s->fsize = rows * s->rs;
+    }
+    int rc = dalloc(s, &s->f, (size_t)s->fsize * ts_size(s->prec));
+    if (rc) return rc;
+    HIPCHK(hipMemset(s->f, 0, (size_t)s->fsize * ts_size(s->prec)));
+    return LB_OK;
+}
+
 int lb_create(const lb_config* cfg, lb_solver** out) {
     if (!cfg || !out) return fail(LB_EINVAL, "null argument");
     *out = nullptr;
@@ -1910,34 +1163,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     s->nq = (uint32_t)((int64_t)s->d.nx * s->d.ny * s->d.nzp);
     s->ncell = (int64_t)s->d.nx * s->d.ny * s->d.nz;
     s->prec = cfg->precision;
-    // The sparse FP32-storage kernels measured 5% faster on the 768x384x384
-    // medium in the plain round-robin order (the sparse FP64 kernel 4% faster
-    // with the XCD order, FP16 the same either way).
-    if (cfg->layout == LB_LAYOUT_SPARSE && (s->prec == LB_FP32_SHIFTED || s->prec == LB_FP32_STORAGE))
-        s->xcd_cells = UINT64_MAX;
-    if (const char* env = getenv("LB_XCD_REMAP_CELLS")) {
-        s->xcd_cells = strtoull(env, nullptr, 10);
-        s->tune_xcd = false;
-    }
-    // Unused dynamic LDS caps the bulk step's occupancy. The FP32 kernel needs
-    // few registers (8 waves per SIMD); its even step streamed 3-4% faster at
-    // 6 waves per SIMD (26 KB per workgroup of 256 threads), while capping the
-    // other variants or the odd step measured slower.
-    if (s->prec == LB_FP32_SHIFTED && cfg->layout != LB_LAYOUT_SPARSE) s->step_lds[0] = 26000;
-    if (const char* env = getenv("LB_STEP_LDS_EVEN")) s->step_lds[0] = (uint32_t)strtoul(env, nullptr, 10);
-    if (const char* env = getenv("LB_STEP_LDS_ODD")) s->step_lds[1] = (uint32_t)strtoul(env, nullptr, 10);
-    for (uint32_t& b : s->step_lds) b = b > 65536u ? 65536u : b;  // 64 KiB keeps one workgroup per CU
-    // Bulk step workgroups: the dense FP64 kernel (4 waves per SIMD) measured
-    // 4% faster with 512 threads on the 256^3 cavity (1.3% slower on the
-    // 512x128x128 duct, even on porous media); FP32 storage measured 8% slower
-    // with 512 on the cavity, and the sparse FP64 odd step holds 3 waves per
-    // SIMD, of which a 512-thread workgroup would leave 2: those and the
-    // rest keep 256. LB_STEP_TILE (a multiple of 64 up to 512) overrides it.
-    if (s->prec == LB_FP64 && cfg->layout != LB_LAYOUT_SPARSE) s->step_tile = 512;
-    if (const char* env = getenv("LB_STEP_TILE")) {
-        const unsigned long t = strtoul(env, nullptr, 10);
-        if (t >= 64 && t <= (unsigned long)STEP_TILE_MAX && t % 64 == 0) s->step_tile = (uint32_t)t;
-    }
+    if (cfg->layout != LB_LAYOUT_AUTO) layout_defaults(s, cfg->layout);
     // Launch tuning on the first long evolve call, for lattices of at least
     // 2^22 cells (below that launch overhead dominates the timings), unless
     // the launch is pinned by LB_STEP_TILE / LB_STEP_LDS_* or LB_AUTOTUNE=0.
@@ -1954,48 +1180,29 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     }
     s->rho0 = cfg->rho0 > 0 ? cfg->rho0 : 1.0;
     s->scale = cfg->fp16_scale > 0 ? cfg->fp16_scale : 1.0;
+    s->f16_auto = !(cfg->fp16_scale > 0);
+    if (const char* env = getenv("LB_FP16_CHECK")) {
+        const unsigned long v = strtoul(env, nullptr, 10);
+        if (v >= 1) s->f16_check = (uint32_t)v;
+    }
     // relaxation parameters, isothermal.pyx:273-287
     const double tau_e = 0.5 + 3.0 * cfg->kvisc;
     const double inv_tau_e = 1.0 / tau_e;
     s->prm[0] = inv_tau_e;
     s->prm[1] = cfg->rlx_scheme == TRT ? 8.0 * (2.0 - inv_tau_e) / (8.0 - inv_tau_e) : 0.0;
-    int8_t mir[3][Q];
-    for (int a = 0; a < 3; ++a)
-        for (int l = 0; l < Q; ++l) mir[a][l] = (int8_t)mirror(l, a);
-    if (hipMemcpyToSymbol(HIP_SYMBOL(c_mirror), mir, sizeof(mir)) != hipSuccess) {
-        delete s;
-        return fail(LB_EHIP, "hipMemcpyToSymbol failed");
-    }
     const size_t nq = s->nq;
-    // population layout: row-interleaved f[i][j][l][k] (default), plain SoA
-    // f[l][i][j][k], or SoA with the population stride padded by 2304 bytes
-    // so that equal offsets of different populations do not share a channel
-    const int64_t rows = (int64_t)s->d.nx * s->d.ny;
-    if (cfg->layout == LB_LAYOUT_SOA || cfg->layout == LB_LAYOUT_SOA_PADDED) {
-        s->rs = s->d.nzp;
-        s->ps = (int64_t)nq + (cfg->layout == LB_LAYOUT_SOA_PADDED ? (int64_t)(2304 / ts_size(s->prec)) : 0);
-    } else {
-        s->rs = (int64_t)Q * s->d.nzp;
-        s->ps = s->d.nzp;
-    }
-    s->fsize = cfg->layout == LB_LAYOUT_SOA || cfg->layout == LB_LAYOUT_SOA_PADDED ? (int64_t)Q * s->ps
-                                                                                  : rows * s->rs;
-    // the sparse layout sizes its arrays from the fluid sites at lb_finalize
-    s->sparse = cfg->layout == LB_LAYOUT_SPARSE;
-    if (s->sparse) s->fsize = 0;
-    if ((!s->sparse && (rc = dalloc(s, &s->f, (size_t)s->fsize * ts_size(s->prec)))) ||
+    if ((cfg->layout != LB_LAYOUT_AUTO && (rc = layout_arrays(s, cfg->layout))) ||
         (rc = dalloc(s, (void**)&s->mask, nq * 4)) || (rc = dalloc(s, (void**)&s->phase, nq)) ||
         (rc = dalloc(s, (void**)&s->wclass, nq / 64)) ||
         (rc = dalloc(s, (void**)&s->den, nq * 8)) || (rc = dalloc(s, (void**)&s->vel, 3 * nq * 8)) ||
-        (rc = dalloc(s, (void**)&s->part, (size_t)RED_PARTS * RED_N * 8)) ||
-        (rc = dalloc(s, (void**)&s->red, RED_N * 8))) {
+        (rc = dalloc(s, (void**)&s->part, (size_t)RED_PARTS * FLOW_N * 8)) ||
+        (rc = dalloc(s, (void**)&s->red, FLOW_N * 8))) {
         std::string msg = g_err;
         lb_destroy(s);
         g_err = msg;
         return rc;
     }
     // den = 1, vel = 0, phase SOLID, mask 0 (isothermal.pyx:243-248)
-    if (s->f) (void)hipMemset(s->f, 0, (size_t)s->fsize * ts_size(s->prec));
     (void)hipMemset(s->mask, 0, nq * 4);
     (void)hipMemset(s->wclass, 1, nq / 64);
     (void)hipMemset(s->phase, 0, nq);
@@ -2035,7 +1242,7 @@ int lb_destroy(lb_solver* s) {
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
                     s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
                     s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab,
-                    s->tabc, s->tabw, s->dseed};
+                    s->tabc, s->tabw, s->dseed, s->st_phase, s->st_den, s->st_vel, s->st_acc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {
@@ -2054,14 +1261,14 @@ int lb_set_stream(lb_solver* s, void* st) {
 
 int lb_set_gravity(lb_solver* s, const double g[3]) {
     if (!s || !g) return fail(LB_EINVAL, "null argument");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     for (int a = 0; a < 3; ++a) s->cfg.g[a] = g[a];
     return LB_OK;
 }
 
 int lb_upload_phase(lb_solver* s, const uint8_t* phase) {
     if (!s || !phase) return fail(LB_EINVAL, "null argument");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     int rc = ensure_stage(s);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s->stage, phase, (size_t)s->ncell, hipMemcpyHostToDevice, s->stream));
@@ -2074,7 +1281,7 @@ int lb_upload_phase(lb_solver* s, const uint8_t* phase) {
 
 int lb_upload_den_vel(lb_solver* s, const double* den, const double* vel) {
     if (!s || !den || !vel) return fail(LB_EINVAL, "null argument");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     int rc = ensure_stage(s);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s->stage, den, (size_t)s->ncell * 8, hipMemcpyHostToDevice, s->stream));
@@ -2087,7 +1294,7 @@ int lb_upload_den_vel(lb_solver* s, const double* den, const double* vel) {
 
 int lb_upload_acc(lb_solver* s, const double* acc) {
     if (!s) return fail(LB_EINVAL, "null handle");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     if (!acc) {
         if (s->acc) {
             HIPCHK(hipStreamSynchronize(s->stream));
@@ -2095,6 +1302,7 @@ int lb_upload_acc(lb_solver* s, const double* acc) {
             s->bytes -= (size_t)3 * s->nq * tc_size(s->prec);
             s->acc = nullptr;
         }
+        s->acc_on = false;
         return LB_OK;
     }
     int rc = ensure_stage(s);
@@ -2107,7 +1315,7 @@ int lb_upload_acc(lb_solver* s, const double* acc) {
 
 int lb_upload_acc_device(lb_solver* s, const double* acc_dev) {
     if (!s || !acc_dev) return fail(LB_EINVAL, "null argument");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     int rc = ensure_stage(s);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s->stage, acc_dev, (size_t)s->ncell * 24, hipMemcpyDeviceToDevice, s->stream));
@@ -2116,7 +1324,7 @@ int lb_upload_acc_device(lb_solver* s, const double* acc_dev) {
 
 int lb_set_face_data(lb_solver* s, int face, const double* den_face, const double* vel_face) {
     if (!s || face < 0 || face > 5 || !den_face || !vel_face) return fail(LB_EINVAL, "invalid argument");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     if (!s->fden[face]) return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", face);
     const size_t n = (size_t)(face < 2 ? s->NJ * s->NK : face < 4 ? s->NI * s->NK : s->NI * s->NJ);
     HIPCHK(hipMemcpyAsync(s->fden[face], den_face, n * 8, hipMemcpyHostToDevice, s->stream));
@@ -2204,7 +1412,7 @@ static int build_sparse(lb_solver* s) {
 
 int lb_finalize(lb_solver* s, int64_t* nfluid) {
     if (!s) return fail(LB_EINVAL, "null handle");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
     s->finalized = false;  // until the populations below are initialised
     int rc = build_mask(s, true);
@@ -2213,8 +1421,19 @@ int lb_finalize(lb_solver* s, int64_t* nfluid) {
     if ((rc = count_mask(s, &nf, &nm))) return rc;
     s->nfluid = nf;
     s->moving = nm > 0;
+    if (s->layout == LB_LAYOUT_AUTO) {
+        // the sparse layout moves the fluid sites' populations only, but reads
+        // 18 B of location codes per odd step: it wins from a few per cent of
+        // solid cells on
+        double thr = LB_AUTO_SPARSE_SOLID;
+        if (const char* env = getenv("LB_AUTO_SPARSE_SOLID")) thr = strtod(env, nullptr);
+        const double solid = 1.0 - (double)nf / (double)s->ncell;
+        const int layout = solid >= thr ? LB_LAYOUT_SPARSE : LB_LAYOUT_ROWS;
+        layout_defaults(s, layout);
+        if ((rc = layout_arrays(s, layout))) return rc;
+    }
     if (s->sparse && (rc = build_sparse(s))) return rc;
-    if (s->prec == LB_FP16_STORAGE && !(s->cfg.fp16_scale > 0)) {
+    if (s->prec == LB_FP16_STORAGE && s->f16_auto) {
         k_maxdev<<<256, 256, 0, s->stream>>>(s->mask, s->den, s->vel, s->nq, s->rho0, s->part);
         HIPCHK(hipGetLastError());
         std::vector<double> h(256);
@@ -2222,17 +1441,21 @@ int lb_finalize(lb_solver* s, int64_t* nfluid) {
         HIPCHK(hipStreamSynchronize(s->stream));
         double mx = 0.0;
         for (double v : h) mx = fmax(mx, v);
+        double fb = 0.0;
+        if ((rc = face_bound(s, &fb))) return rc;
         // populations are O(w * mx) with w <= 1/3; keep them near 2^10 so that the
-        // f16 range (6.1e-5 .. 65504) leaves ~64x headroom for growth
-        const double est = fmax(mx / 3.0, 1e-4);
+        // f16 range (6.1e-5 .. 65504) leaves ~64x headroom for growth, which the
+        // guard (fp16_guard) watches from then on
+        const double est = fmax(fmax(mx, fb) / 3.0, 1e-4);
         s->scale = ldexp(1.0, (int)floor(log2(1024.0 / est)));
         s->scale = fmin(fmax(s->scale, 1.0), ldexp(1.0, 24));
     }
+    s->f16_since = 0;
     switch (s->prec) {
-    case LB_FP64: launch_init<LB_FP64>(s); break;
-    case LB_FP32_SHIFTED: launch_init<LB_FP32_SHIFTED>(s); break;
-    case LB_FP32_STORAGE: launch_init<LB_FP32_STORAGE>(s); break;
-    default: launch_init<LB_FP16_STORAGE>(s); break;
+    case LB_FP64: launch_init<LB_FP64>(params(s), s->stream); break;
+    case LB_FP32_SHIFTED: launch_init<LB_FP32_SHIFTED>(params(s), s->stream); break;
+    case LB_FP32_STORAGE: launch_init<LB_FP32_STORAGE>(params(s), s->stream); break;
+    default: launch_init<LB_FP16_STORAGE>(params(s), s->stream); break;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -2246,7 +1469,7 @@ int lb_finalize(lb_solver* s, int64_t* nfluid) {
 
 int lb_refresh_walls(lb_solver* s) {
     if (!s) return fail(LB_EINVAL, "null handle");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
     int rc = build_mask(s, true);
     if (rc) return rc;
@@ -2256,16 +1479,87 @@ int lb_refresh_walls(lb_solver* s) {
     return LB_OK;
 }
 
+// Zou-He data bound of |f - w rho0| over the FIXED faces (k_face_bound), 0
+// without such faces
+static int face_bound(lb_solver* s, double* out) {
+    *out = 0.0;
+    int nf = 0;
+    for (int f = 0; f < 6; ++f) {
+        if (!s->fden[f]) continue;
+        const size_t n = (size_t)(f < 2 ? s->NJ * s->NK : f < 4 ? s->NI * s->NK : s->NI * s->NJ);
+        k_face_bound<<<64, 256, 0, s->stream>>>(s->fden[f], s->fvel[f], n, s->rho0, s->part + 64 * nf);
+        ++nf;
+    }
+    if (!nf) {
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return LB_OK;
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(64 * nf);
+    HIPCHK(hipMemcpyAsync(h.data(), s->part, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (double v : h) *out = fmax(*out, v);
+    return LB_OK;
+}
+
+static double half_bits_value(uint32_t b) {
+    const int e = (int)((b >> 10) & 0x1f), m = (int)(b & 0x3ff);
+    return e ? ldexp(1024.0 + m, e - 25) : ldexp((double)m, -24);
+}
+
+// FP16 storage guard. The stored values are s (f - w rho0). Their largest
+// magnitude, and the bound the Zou-He face data imply, are kept at or below
+// 2^13 (and, with an automatic scale, at or above 2^6) by exact power-of-two
+// rescales of the whole array, so that a flow which accelerates (gravity, an
+// inlet switched on) does not overflow the f16 range (65504) and one that
+// stays slow keeps its precision. Runs at the start of every evolve call and
+// every f16_check steps within one. A non-finite stored value (an overflow
+// between two checks) is an error, never a silent inf.
+static int fp16_guard(lb_solver* s) {
+    if (s->prec != LB_FP16_STORAGE || !s->f) return LB_OK;
+    unsigned int* d = (unsigned int*)s->red;
+    HIPCHK(hipMemsetAsync(d, 0, 4, s->stream));
+    const size_t n8 = (size_t)s->fsize / 8;  // fsize is a multiple of 64
+    k_f16_amax<<<1024, 256, 0, s->stream>>>((const uint4*)s->f, n8, d);
+    HIPCHK(hipGetLastError());
+    unsigned int bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, d, 4, hipMemcpyDeviceToHost, s->stream));
+    double fb = 0.0;
+    int rc = face_bound(s, &fb);  // synchronises the stream
+    if (rc) return rc;
+    s->f16_since = 0;
+    if (bits >= 0x7c00u)
+        return fail(LB_ESTATE, "FP16 storage overflowed: non-finite populations after step %llu (scale %g); "
+                    "use a smaller fp16_scale or FP32", (unsigned long long)s->steps, s->scale);
+    const double m = fmax(half_bits_value(bits), s->scale * fb / 3.0);
+    int e = 0;
+    if (m > 8192.0) e = 10 - (int)ceil(log2(m));
+    else if (s->f16_auto && m > 0.0 && m < 64.0) e = std::min(10 - (int)ceil(log2(m)), 24 - (int)lround(log2(s->scale)));
+    if (e == 0) return LB_OK;
+    k_f16_scale<<<1024, 256, 0, s->stream>>>((_Float16*)s->f, (size_t)s->fsize, ldexpf(1.0f, e));
+    HIPCHK(hipGetLastError());
+    s->scale = ldexp(s->scale, e);
+    return LB_OK;
+}
+
 // `n` steps with the solver's current launch configuration; the last one
-// stores den/vel if `store_last`.
-static void run_steps(lb_solver* s, uint32_t n, bool store_last) {
+// stores den/vel if `store_last`. FP16 storage runs its guard every
+// f16_check steps.
+static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
     StepParams p = params(s);
     for (uint32_t t = 0; t < n; ++t) {
+        if (s->prec == LB_FP16_STORAGE && s->f16_since >= s->f16_check) {
+            int rc = fp16_guard(s);
+            if (rc) return rc;
+            p = params(s);
+        }
         p.store_dv = (store_last && t + 1 == n) ? 1 : 0;
         launch_any_step(s, p, s->next_odd != 0);
         s->next_odd ^= 1;
         s->steps++;
+        s->f16_since++;
     }
+    return LB_OK;
 }
 
 // Launch configurations of the bulk step that the first long evolve call
@@ -2328,15 +1622,19 @@ static int tune_step(lb_solver* s) {
                 if (x) (void)hipEventDestroy(x);
             return LB_OK;  // no timing: keep the default
         }
-    run_steps(s, TUNE_WARM, false);
-    for (int i = 0; i < n; ++i) {
+    int rc = run_steps(s, TUNE_WARM, false);
+    for (int i = 0; i < n && !rc; ++i) {
         s->step_tile = c[i].tile;
         s->step_lds[0] = c[i].lds_even;
         s->step_lds[1] = c[i].lds_odd;
         s->xcd_cells = c[i].xcd_cells;
         HIPCHK(hipEventRecord(ev[2 * i], s->stream));
-        run_steps(s, TUNE_STEPS, false);
+        rc = run_steps(s, TUNE_STEPS, false);
         HIPCHK(hipEventRecord(ev[2 * i + 1], s->stream));
+    }
+    if (rc) {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        return rc;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(ev[2 * n - 1]));
@@ -2367,10 +1665,11 @@ static int tune_step(lb_solver* s) {
     return LB_OK;
 }
 
-int lb_evolve(lb_solver* s, uint32_t steps) {
-    if (!s) return fail(LB_EINVAL, "null handle");
-    if (!s->finalized) return fail(LB_ESTATE, "Finalize solver initialization before calling evolve");
-    if (s->nfluid < 1 || steps == 0) return LB_OK;
+// The steps of one evolve call: the FP16 guard, the launch tuning of the
+// first long call, then the steps (den/vel stored on the last one).
+static int evolve_steps(lb_solver* s, uint32_t steps) {
+    int rc = fp16_guard(s);
+    if (rc) return rc;
     if (s->tune) {
         // tuning takes TUNE_WARM + TUNE_STEPS per candidate of this call's
         // steps; a shorter call leaves it to a later one
@@ -2379,13 +1678,198 @@ int lb_evolve(lb_solver* s, uint32_t steps) {
         if (n < 2) s->tune = false;
         else if (steps > (uint32_t)(TUNE_WARM + n * TUNE_STEPS)) {
             const uint64_t before = s->steps;
-            int rc = tune_step(s);
-            if (rc) return rc;
+            if ((rc = tune_step(s))) return rc;
             steps -= (uint32_t)(s->steps - before);
         }
     }
-    run_steps(s, steps, true);
+    if ((rc = run_steps(s, steps, true))) return rc;
     HIPCHK(hipGetLastError());
+    return LB_OK;
+}
+
+int lb_evolve(lb_solver* s, uint32_t steps) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->finalized) return fail(LB_ESTATE, "Finalize solver initialization before calling evolve");
+    inputs_changed(s);  // the state moves on: an earlier adjoint forward pass no longer applies
+    if (s->nfluid < 1 || steps == 0) return LB_OK;
+    return evolve_steps(s, steps);
+}
+
+int lb_host_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return fail(LB_EINVAL, "null argument");
+    *ptr = nullptr;
+    if (bytes == 0) return LB_OK;
+    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(LB_ENOMEM, "hipHostMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return LB_OK;
+}
+
+int lb_host_free(void* ptr) {
+    if (ptr) HIPCHK(hipHostFree(ptr));
+    return LB_OK;
+}
+
+int lb_bind_host_fields(lb_solver* s, uint8_t* phase, double* den, double* vel, double* acc) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (s->stream) HIPCHK(hipStreamSynchronize(s->stream));
+    const int64_t ncnt = (int64_t)s->NI * s->NJ * s->NK;
+    if (!phase && !den && !vel && !acc) {  // unbind
+        s->h_phase = nullptr;
+        s->h_den = s->h_vel = s->h_acc = nullptr;
+        return LB_OK;
+    }
+    if (!phase || !den || !vel || !acc) return fail(LB_EINVAL, "lb_bind_host_fields: all four fields are needed");
+    int rc;
+    if (!s->st_phase) {
+        if ((rc = dalloc(s, (void**)&s->st_phase, (size_t)ncnt)) ||
+            (rc = dalloc(s, (void**)&s->st_den, (size_t)ncnt * 8)) ||
+            (rc = dalloc(s, (void**)&s->st_vel, (size_t)ncnt * 24)) ||
+            (rc = dalloc(s, (void**)&s->st_acc, (size_t)ncnt * 24)))
+            return rc;
+    }
+    if (!s->acc && (rc = dalloc(s, &s->acc, (size_t)3 * s->nq * tc_size(s->prec)))) return rc;
+    s->ncnt = ncnt;
+    s->h_phase = phase;
+    s->h_den = den;
+    s->h_vel = vel;
+    s->h_acc = acc;
+    return LB_OK;
+}
+
+static bool has_halo_faces(const lb_solver* s) {
+    for (int f = 0; f < 6; ++f)
+        if (s->cfg.bc[f] == PERIODIC || s->cfg.bc[f] == SYMMETRIC) return true;
+    return false;
+}
+
+// den/vel of the fluid sites into the staged host fields, optionally the
+// den/vel halo pass there
+static int scatter_staged(lb_solver* s, bool halo) {
+    k_scatter_fluid<<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(s->mask, s->den, s->vel, s->st_den, s->st_vel,
+                                                                   s->d, s->nq);
+    HIPCHK(hipGetLastError());
+    if (halo) {
+        const int64_t n[3] = {(int64_t)s->d.ny * s->d.nz, (int64_t)s->NI * s->d.nz, (int64_t)s->NI * s->NJ};
+        for (int a = 0; a < 3; ++a) {
+            if (s->cfg.bc[2 * a] != PERIODIC && s->cfg.bc[2 * a] != SYMMETRIC && s->cfg.bc[2 * a + 1] != PERIODIC &&
+                s->cfg.bc[2 * a + 1] != SYMMETRIC)
+                continue;
+            k_halo_dv<<<(unsigned)((n[a] + 255) / 256), 256, 0, s->stream>>>(s->st_den, s->st_vel, s->d, a);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    return LB_OK;
+}
+
+static int halo_only(lb_solver* s) {
+    const int64_t n[3] = {(int64_t)s->d.ny * s->d.nz, (int64_t)s->NI * s->d.nz, (int64_t)s->NI * s->NJ};
+    for (int a = 0; a < 3; ++a)
+        k_halo_dv<<<(unsigned)((n[a] + 255) / 256), 256, 0, s->stream>>>(s->st_den, s->st_vel, s->d, a);
+    HIPCHK(hipGetLastError());
+    return LB_OK;
+}
+
+// The host fields as they are now (the user may have written any of them):
+// staged on the device, checked against the finalized phase, and taken as
+// the inputs of the next steps (k_sync_in).
+static int load_host_fields(lb_solver* s) {
+    const int64_t n = s->ncnt;
+    HIPCHK(hipMemcpyAsync(s->st_phase, s->h_phase, (size_t)n, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->st_den, s->h_den, (size_t)n * 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->st_vel, s->h_vel, (size_t)n * 24, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->st_acc, s->h_acc, (size_t)n * 24, hipMemcpyHostToDevice, s->stream));
+    unsigned int* flags = (unsigned int*)s->red;
+    HIPCHK(hipMemsetAsync(flags, 0, 4, s->stream));
+    if (tc_size(s->prec) == 8)
+        k_sync_in<double><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(
+            s->st_phase, s->st_den, s->st_vel, s->st_acc, s->phase, s->den, s->vel, (double*)s->acc, s->d, s->nq,
+            flags);
+    else
+        k_sync_in<float><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(
+            s->st_phase, s->st_den, s->st_vel, s->st_acc, s->phase, s->den, s->vel, (float*)s->acc, s->d, s->nq,
+            flags);
+    HIPCHK(hipGetLastError());
+    unsigned int hf = 0;
+    HIPCHK(hipMemcpyAsync(&hf, flags, 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (hf & SYNC_PHASE)
+        return fail(LB_ESTATE, "the phase field was changed after finalize_init; the engine builds its links "
+                    "from the phase at finalize_init, so phase edits after it are not supported");
+    if (hf & SYNC_WALLS) {  // wall velocities changed: moving-wall flags and Ladd table
+        int rc = build_mask(s, true);
+        if (rc) return rc;
+        int64_t nf, nm;
+        if ((rc = count_mask(s, &nf, &nm))) return rc;
+        s->moving = nm > 0;
+    }
+    s->acc_on = (hf & SYNC_ACC) != 0;
+    return LB_OK;
+}
+
+int lb_load_host_fields(lb_solver* s) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
+    if (!s->h_phase) return fail(LB_ESTATE, "no host fields bound (lb_bind_host_fields)");
+    inputs_changed(s);
+    return load_host_fields(s);
+}
+
+int lb_evolve_host(lb_solver* s, uint32_t steps) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->finalized) return fail(LB_ESTATE, "Finalize solver initialization before calling evolve");
+    if (!s->h_phase) return fail(LB_ESTATE, "no host fields bound (lb_bind_host_fields)");
+    inputs_changed(s);
+    if (s->nfluid < 1 || steps == 0) return LB_OK;
+    int rc = load_host_fields(s);
+    if (rc) return rc;
+    // Isothermal.evolve: the halo den/vel pass precedes every step, so after
+    // the call the halos hold the state of step T-1 (isothermal.pyx:1408-1415)
+    const bool halo = has_halo_faces(s);
+    if (halo && steps >= 2) {
+        if ((rc = evolve_steps(s, steps - 1))) return rc;
+        if ((rc = scatter_staged(s, true))) return rc;
+    } else if (halo && (rc = halo_only(s))) {
+        return rc;
+    }
+    if ((rc = evolve_steps(s, halo ? 1 : steps))) return rc;
+    if ((rc = scatter_staged(s, false))) return rc;
+    if (s->prec == LB_FP16_STORAGE && (rc = fp16_guard(s))) return rc;  // no silent inf in the fields
+    const int64_t n = s->ncnt;
+    HIPCHK(hipMemcpyAsync(s->h_den, s->st_den, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_vel, s->st_vel, (size_t)n * 24, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_store_host_fields(lb_solver* s, int halo_pass) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    if (!s->h_phase) return fail(LB_ESTATE, "no host fields bound (lb_bind_host_fields)");
+    const int64_t n = s->ncnt;
+    // the SOLID sites and halos keep the host's values
+    HIPCHK(hipMemcpyAsync(s->st_den, s->h_den, (size_t)n * 8, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->st_vel, s->h_vel, (size_t)n * 24, hipMemcpyHostToDevice, s->stream));
+    int rc = scatter_staged(s, halo_pass != 0 && has_halo_faces(s));
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->h_den, s->st_den, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(s->h_vel, s->st_vel, (size_t)n * 24, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return LB_OK;
+}
+
+int lb_flow_info(lb_solver* s, lb_flow_info_t* info) {
+    if (!s || !info) return fail(LB_EINVAL, "null argument");
+    const int nb = (int)std::min<uint32_t>(RED_PARTS, blocks_for(s->nq, RED_BLOCK));
+    SolverSites sites{s->mask, s->den, s->vel, s->nq};
+    k_flow1<SolverSites><<<nb, RED_BLOCK, 0, s->stream>>>(sites, s->rho0, s->part);
+    k_flow2<<<1, RED_BLOCK, 0, s->stream>>>(s->part, nb, s->rho0, s->red);
+    HIPCHK(hipGetLastError());
+    double r[FLOW_N];
+    HIPCHK(hipMemcpyAsync(r, s->red, FLOW_N * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    flow_info_from(r, info);
     return LB_OK;
 }
 
@@ -2420,15 +1904,15 @@ int lb_download_post_f(lb_solver* s, double* f) {
     if (rc) return rc;
     StepParams p = params(s);
     switch (s->prec) {
-    case LB_FP64: k_post_f<LB_FP64><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf); break;
+    case LB_FP64: launch_post_f<LB_FP64>(p, s->next_odd, buf, s->stream); break;
     case LB_FP32_SHIFTED:
-        k_post_f<LB_FP32_SHIFTED><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
+        launch_post_f<LB_FP32_SHIFTED>(p, s->next_odd, buf, s->stream);
         break;
     case LB_FP32_STORAGE:
-        k_post_f<LB_FP32_STORAGE><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
+        launch_post_f<LB_FP32_STORAGE>(p, s->next_odd, buf, s->stream);
         break;
     default:
-        k_post_f<LB_FP16_STORAGE><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>(p, s->next_odd, buf);
+        launch_post_f<LB_FP16_STORAGE>(p, s->next_odd, buf, s->stream);
         break;
     }
     hipError_t e = hipGetLastError();
@@ -2559,8 +2043,8 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
                 q.rec = nullptr;
                 q.dseed = s->dseed;
                 q.dscale = 1.0 / (double)s->nfluid;
-                if (s->prec == LB_FP64) launch_diss_rlx<LB_FP64>(s, q, s->next_odd != 0);
-                else launch_diss_rlx<LB_FP32_SHIFTED>(s, q, s->next_odd != 0);
+                if (s->prec == LB_FP64) launch_diss_rlx<LB_FP64>(q, s->cfg.rlx_scheme, s->next_odd != 0, s->stream);
+                else launch_diss_rlx<LB_FP32_SHIFTED>(q, s->cfg.rlx_scheme, s->next_odd != 0, s->stream);
             }
             launch_any_step(s, p, s->next_odd != 0);
             s->next_odd ^= 1;
@@ -2601,7 +2085,12 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
 
 static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* grad_face, bool device) {
     if (!s) return fail(LB_EINVAL, "null handle");
-    if (!s->adj_ready) return fail(LB_ESTATE, "lb_adj_forward must precede lb_adj_backward");
+    if (!s->adj_ready)
+        return fail(LB_ESTATE, "lb_adj_backward needs the lb_adj_forward pass of the current state and inputs: "
+                    "an evolve step, an input upload or a restore since the forward pass invalidates it");
+    if (s->steps != s->adj_step_end)
+        return fail(LB_ESTATE, "the state advanced since lb_adj_forward (step %llu, forward pass ended at %llu)",
+                    (unsigned long long)s->steps, (unsigned long long)s->adj_step_end);
     if (gface_id >= 0 && (gface_id > 5 || !s->fden[gface_id]))
         return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", gface_id);
     const size_t tcs = tc_size(s->prec);
@@ -2660,8 +2149,8 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
             x.rec = (const char*)s->rec + (size_t)t * 4 * s->nq * tcs;
             x.seed = (g + 1 == steps) ? 1 : 0;
             const bool odd = (s->adj_po ^ (g & 1)) != 0;
-            if (s->prec == LB_FP64) launch_adj_rlx<LB_FP64>(s, p, x, odd);
-            else launch_adj_rlx<LB_FP32_SHIFTED>(s, p, x, odd);
+            if (s->prec == LB_FP64) launch_adj_rlx<LB_FP64>(p, x, s->cfg.rlx_scheme, odd, s->stream);
+            else launch_adj_rlx<LB_FP32_SHIFTED>(p, x, s->cfg.rlx_scheme, odd, s->stream);
         }
         HIPCHK(hipGetLastError());
     }
@@ -2694,7 +2183,7 @@ int lb_adj_backward_device(lb_solver* s, int face, double* grad_acc_dev, double*
 
 int lb_set_face_data_device(lb_solver* s, int face, const double* den_dev, const double* vel_dev) {
     if (!s || face < 0 || face > 5 || !den_dev || !vel_dev) return fail(LB_EINVAL, "invalid argument");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     if (!s->fden[face]) return fail(LB_EINVAL, "face %d has no FIXED_DEN/FIXED_VEL condition", face);
     const size_t n = (size_t)(face < 2 ? s->NJ * s->NK : face < 4 ? s->NI * s->NK : s->NI * s->NJ);
     HIPCHK(hipMemcpyAsync(s->fden[face], den_dev, n * 8, hipMemcpyDeviceToDevice, s->stream));
@@ -2720,7 +2209,7 @@ int lb_snapshot(lb_solver* s) {
 
 int lb_restore(lb_solver* s) {
     if (!s) return fail(LB_EINVAL, "null handle");
-    s->rec_fwd = false;  // inputs change: the reverse pass recomputes every segment
+    inputs_changed(s);
     if (!s->snap) return fail(LB_ESTATE, "no snapshot taken");
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
     HIPCHK(hipMemcpyAsync(s->f, s->snap, fbytes, hipMemcpyDeviceToDevice, s->stream));
@@ -2742,6 +2231,12 @@ int lb_info(lb_solver* s, uint64_t* steps, int* next_odd, uint64_t* bytes, doubl
     return LB_OK;
 }
 
+int lb_layout(lb_solver* s, int* layout) {
+    if (!s || !layout) return fail(LB_EINVAL, "null argument");
+    *layout = s->layout;
+    return LB_OK;
+}
+
 int lb_synchronize(lb_solver* s) {
     if (!s) return fail(LB_EINVAL, "null handle");
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -2749,3 +2244,4 @@ int lb_synchronize(lb_solver* s) {
 }
 
 }  // extern "C"
+

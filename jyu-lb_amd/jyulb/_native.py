@@ -19,9 +19,10 @@ PRECISIONS = {"fp64": LB_FP64, "fp32": LB_FP32_SHIFTED, "fp32_shifted": LB_FP32_
               LB_FP32_STORAGE: LB_FP32_STORAGE}
 
 
-LB_LAYOUT_ROWS, LB_LAYOUT_SOA, LB_LAYOUT_SOA_PADDED, LB_LAYOUT_SPARSE = 0, 1, 2, 3
+LB_LAYOUT_ROWS, LB_LAYOUT_SOA, LB_LAYOUT_SOA_PADDED, LB_LAYOUT_SPARSE, LB_LAYOUT_AUTO = 0, 1, 2, 3, 4
 LAYOUTS = {"rows": LB_LAYOUT_ROWS, "soa": LB_LAYOUT_SOA, "soa_padded": LB_LAYOUT_SOA_PADDED,
-           "sparse": LB_LAYOUT_SPARSE}
+           "sparse": LB_LAYOUT_SPARSE, "auto": LB_LAYOUT_AUTO}
+LAYOUT_NAMES = {v: k for k, v in LAYOUTS.items()}
 LAYOUTS.update({v: v for v in list(LAYOUTS.values())})
 
 
@@ -37,6 +38,14 @@ class LbConfig(ctypes.Structure):
         ("fp16_scale", ctypes.c_double),
         ("layout", ctypes.c_int32),
     ]
+
+
+class LbFlowInfo(ctypes.Structure):
+    """lb_flow_info_t: the fields of the reference's IsothermalFlowInfo
+    (include/solver/solver.h:13-15) plus the number of fluid sites."""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "tot_jx", "tot_jy", "tot_jz", "tot_j2", "tot_den", "ave_ux", "ave_uy", "ave_uz", "ave_u2",
+        "max_u2", "max_den", "min_den", "fluid_sites")]
 
 
 _lib = None
@@ -90,6 +99,15 @@ def lib():
         "lb_adj_backward_device": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "lb_snapshot": ([H], ctypes.c_int),
         "lb_restore": ([H], ctypes.c_int),
+        "lb_flow_info": ([H, ctypes.POINTER(LbFlowInfo)], ctypes.c_int),
+        "lb_host_alloc": ([ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "lb_host_free": ([ctypes.c_void_p], ctypes.c_int),
+        "lb_bind_host_fields": ([H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+                                ctypes.c_int),
+        "lb_load_host_fields": ([H], ctypes.c_int),
+        "lb_evolve_host": ([H, ctypes.c_uint32], ctypes.c_int),
+        "lb_store_host_fields": ([H, ctypes.c_int], ctypes.c_int),
+        "lb_layout": ([H, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -113,10 +131,44 @@ def ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+class _Pinned:
+    """Owner of one pinned host allocation (lb_host_alloc); numpy arrays made
+    from it keep it alive through their base, and it is freed with the last."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().lb_host_alloc(ctypes.c_size_t(int(nbytes)), ctypes.byref(p)))
+        self.ptr = p.value
+        self.__array_interface__ = {"shape": (int(nbytes),), "typestr": "|u1", "data": (self.ptr, False),
+                                    "version": 3}
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            try:
+                lib().lb_host_free(ctypes.c_void_p(self.ptr))
+            except Exception:
+                pass
+            self.ptr = None
+
+
+def host_array(shape, dtype, fill):
+    """C-contiguous array of `shape` filled with `fill`, in pinned host memory
+    when it can be had (plain numpy memory otherwise: same results, slower
+    copies)."""
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    try:
+        a = np.asarray(_Pinned(max(n, 1)))[:n].view(dtype).reshape(shape)
+    except (RuntimeError, ValueError, MemoryError):
+        return np.full(shape, fill, dtype=dtype)
+    a[...] = fill
+    return a
+
+
 class Engine:
     """Owner of one device solver handle (lb_solver*)."""
 
-    def __init__(self, lat, rlx, kvisc, g, bc, precision=LB_FP64, rho0=1.0, fp16_scale=0.0, layout=LB_LAYOUT_ROWS):
+    def __init__(self, lat, rlx, kvisc, g, bc, precision=LB_FP64, rho0=1.0, fp16_scale=0.0, layout=LB_LAYOUT_AUTO):
         if precision not in PRECISIONS:
             raise ValueError(f"unknown precision {precision!r}; one of "
                              f"{sorted(k for k in PRECISIONS if isinstance(k, str))}")
@@ -209,6 +261,31 @@ class Engine:
         self.call("lb_download_post_f", ptr(f))
         return f
 
+    def flow_info(self):
+        """calc_flow_info on the device (lb_flow_info): dict of the
+        IsothermalFlowInfo fields plus fluid_sites."""
+        info = LbFlowInfo()
+        self.call("lb_flow_info", ctypes.byref(info))
+        return {n: float(getattr(info, n)) for n, _ in LbFlowInfo._fields_}
+
+    def bind_host_fields(self, phase, den, vel, acc):
+        """Bind the Solver's halo-padded host arrays (kept referenced here, so
+        that they outlive the binding)."""
+        for a, dt in ((phase, np.uint8), (den, np.float64), (vel, np.float64), (acc, np.float64)):
+            if a.dtype != dt or not a.flags.c_contiguous:
+                raise ValueError("host fields must be C-contiguous arrays of the Solver's dtypes")
+        self.call("lb_bind_host_fields", ptr(phase), ptr(den), ptr(vel), ptr(acc))
+        self._bound = (phase, den, vel, acc)
+
+    def load_host_fields(self):
+        self.call("lb_load_host_fields")
+
+    def evolve_host(self, steps):
+        self.call("lb_evolve_host", ctypes.c_uint32(int(steps)))
+
+    def store_host_fields(self, halo_pass=True):
+        self.call("lb_store_host_fields", int(bool(halo_pass)))
+
     def reduce(self):
         out = np.zeros(8)
         self.call("lb_reduce", out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
@@ -223,6 +300,12 @@ class Engine:
         st, odd, by, sc = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint64(), ctypes.c_double()
         self.call("lb_info", ctypes.byref(st), ctypes.byref(odd), ctypes.byref(by), ctypes.byref(sc))
         return dict(steps=st.value, next_odd=odd.value, device_bytes=by.value, fp16_scale=sc.value)
+
+    def layout(self):
+        """Name of the population layout in use ("auto" until finalize resolves it)."""
+        v = ctypes.c_int()
+        self.call("lb_layout", ctypes.byref(v))
+        return LAYOUT_NAMES[v.value]
 
     def synchronize(self):
         self.call("lb_synchronize")
@@ -239,7 +322,10 @@ class Engine:
         self.call("lb_objective", int(face), int(axis), ctypes.byref(out))
         return out.value
 
+    adj_gen = 0  # number of adjoint forward passes run on this handle
+
     def adj_forward(self, steps, interval, face, axis):
+        self.adj_gen += 1
         out = ctypes.c_double()
         self.call("lb_adj_forward", ctypes.c_uint32(int(steps)), ctypes.c_uint32(int(interval)),
                   int(face), int(axis), ctypes.byref(out))

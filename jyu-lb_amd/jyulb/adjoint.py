@@ -35,25 +35,28 @@ def _prepare(solver):
     if not solver.init_done or solver.fluid_lsite_domain is None:
         raise RuntimeError('Finalize solver initialization before differentiating evolve')
     solver._set_bc_data()
-    solver._sync_inputs()
+    solver._load_inputs()
 
 
 def _finish(solver, steps):
     solver._pull_fields()
-    solver._enforce_den_vel_per_sym_bc()
     solver.tot_tsteps = (solver.tot_tsteps + int(steps)) & 0xFFFFFFFF
 
 
-def checkpoint_interval(solver, steps, budget_bytes=None):
-    """The largest checkpoint interval whose checkpoints and (rho, u*) record
-    fit in `budget_bytes` (default: half of the free device memory).
+def checkpoint_interval(solver, steps, budget_bytes=None, objective="flux"):
+    """The checkpoint interval with the least recomputation whose device
+    memory fits in `budget_bytes` (default: half of the free device memory).
 
-    The reverse pass recomputes all but the last segment, so a larger interval
-    is faster; interval = steps recomputes nothing. Memory for interval i:
-    (ceil(steps / i) + 1) population arrays (19 values per site) plus
-    min(i, steps) x 4 values per site, with nx * ny * round_up(nz, 64) sites
-    and 8 bytes per value in FP64, 4 in FP32. Raises MemoryError if even the
-    smallest footprint does not fit.
+    With n segments of length i = ceil(steps / n) the reverse pass recomputes
+    all but the last segment; the fewest segments that fit are the fastest
+    choice, and i = steps (one segment) recomputes nothing. Memory for
+    interval i: (ceil(steps / i) + 1) population arrays (19 values per site)
+    for the checkpoints, min(i, steps) x 4 values per site for the (rho, u*)
+    record, one population array of cotangents and 3 values per site of the
+    acceleration cotangent, plus (Q + 4) values per site of seeds for the
+    dissipation objective; nx * ny * round_up(nz, 64) sites and 8 bytes per
+    value in FP64, 4 in FP32. Raises MemoryError if even the smallest
+    footprint does not fit.
     """
     from ._native import LB_FP64
     eng = solver.engine
@@ -61,6 +64,7 @@ def checkpoint_interval(solver, steps, budget_bytes=None):
     nq = int(nx) * int(ny) * ((int(nz) + 63) // 64 * 64)
     vb = 8 if eng.precision == LB_FP64 else 4
     fb, rb = 19 * nq * vb, 4 * nq * vb
+    extra = fb + 3 * nq * vb + (23 * nq * vb if objective == "dissipation" else 0)
     steps = int(steps)
     if steps < 1:
         raise ValueError('steps must be >= 1')
@@ -69,17 +73,18 @@ def checkpoint_interval(solver, steps, budget_bytes=None):
         budget_bytes = torch.cuda.mem_get_info()[0] // 2
 
     def need(i):
-        return (-(-steps // i) + 1) * fb + i * rb
+        return (-(-steps // i) + 1) * fb + min(i, steps) * rb + extra
 
-    for i in range(steps, 0, -1):
+    for nseg in range(1, steps + 1):
+        i = -(-steps // nseg)
         if need(i) <= budget_bytes:
             return i
     raise MemoryError(f'the adjoint of {steps} steps needs at least '
                       f'{min(need(i) for i in range(1, steps + 1))} bytes; {budget_bytes} available')
 
 
-def _interval(solver, steps, interval):
-    return checkpoint_interval(solver, steps) if interval == 'auto' else int(interval)
+def _interval(solver, steps, interval, objective="flux"):
+    return checkpoint_interval(solver, steps, objective=objective) if interval == 'auto' else int(interval)
 
 
 def flux_gradient(solver, steps, interval=50, face=FACE_X1, axis=X, grad_face=None):
@@ -102,7 +107,7 @@ def dissipation_gradient(solver, steps, interval=50, grad_face=None):
     """Like flux_gradient, for the dissipation objective."""
     _prepare(solver)
     eng = solver.engine
-    J = eng.adj_forward(steps, _interval(solver, steps, interval), OBJ_DISSIPATION, 0)
+    J = eng.adj_forward(steps, _interval(solver, steps, interval, 'dissipation'), OBJ_DISSIPATION, 0)
     ga, gf = eng.adj_backward(-1 if grad_face is None else int(grad_face))
     _finish(solver, steps)
     return J, ga, gf
@@ -145,6 +150,7 @@ def _torch_function():
             J = eng.adj_forward(steps, interval, face, axis)
             _finish(solver, steps)
             ctx.solver = solver
+            ctx.gen = eng.adj_gen  # this forward pass; backward() must follow it
             ctx.inlet_face = inlet_face if face_vel is not None else -1
             ctx.acc_shape = acc.shape
             ctx.face_shape = None if face_vel is None else face_vel.shape
@@ -154,6 +160,10 @@ def _torch_function():
         @staticmethod
         def backward(ctx, gJ):
             eng = ctx.solver.engine
+            if eng.adj_gen != ctx.gen:
+                raise RuntimeError('the solver ran another adjoint forward pass after this objective was '
+                                   'evaluated: its reverse pass needs its own forward pass (one objective per '
+                                   'solver per backward)')
             eng.set_stream(torch.cuda.current_stream().cuda_stream)
             ga = torch.empty(ctx.acc_shape, dtype=torch.float64, device=ctx.device)
             gf = None
@@ -180,8 +190,9 @@ def flux_objective(acc, face_vel, solver, steps, interval=50, face=FACE_X1, axis
         _FN = _torch_function()
     if face_vel is not None and inlet_face is None:
         inlet_face = fixed_face(solver)
-    return _FN.apply(acc, face_vel, solver, int(steps), _interval(solver, steps, interval), int(face), int(axis),
-                     -1 if inlet_face is None else int(inlet_face))
+    obj = "dissipation" if int(face) == OBJ_DISSIPATION else "flux"
+    return _FN.apply(acc, face_vel, solver, int(steps), _interval(solver, steps, interval, obj), int(face),
+                     int(axis), -1 if inlet_face is None else int(inlet_face))
 
 
 def dissipation_objective(acc, face_vel, solver, steps, interval=50, inlet_face=None):

@@ -18,10 +18,11 @@ Additions, keyword-only so that reference call sites are unaffected:
              storage of the scaled shifted populations, FP32 arithmetic) or
              "fp32_storage" (FP32 storage of the shifted populations, FP64
              arithmetic: the FP32 traffic with the rounding of storage only)
-  layout     population layout on the device: "rows" (default), "soa",
-             "soa_padded", or "sparse" (fluid sites only, with a table of the
-             odd-step locations: less traffic on porous media).
-             Results are identical in every layout.
+  layout     population layout on the device: "auto" (default: chosen at
+             finalize_init, "sparse" when at least 5 % of the cells are solid,
+             "rows" otherwise), "rows", "soa", "soa_padded", or "sparse"
+             (fluid sites only, with a table of the odd-step locations: less
+             traffic on porous media). Results are identical in every layout.
 """
 import numpy as np
 
@@ -43,7 +44,7 @@ class Solver:
     # constructors
     # ------------------------------------------------------------------
     def __init__(self, lat_size, rlx_scheme=BGK, kvisc=1.0 / 6.0, g=None, bc=None, *,
-                 precision="fp64", rho0=1.0, fp16_scale=0.0, layout="rows"):
+                 precision="fp64", rho0=1.0, fp16_scale=0.0, layout="auto"):
         """isothermal.pyx:156-298"""
         self.error_msg = ''
         if lat_size is None:
@@ -68,12 +69,19 @@ class Solver:
         NI, NJ, NK = int(lat_size[X]) + 2, int(lat_size[Y]) + 2, int(lat_size[Z]) + 2
         self.NI, self.NJ, self.NK, self.NCNT = NI, NJ, NK, NI * NJ * NK
 
-        # flow field, isothermal.pyx:243-248
-        self.phase = np.full((NI, NJ, NK), SOLID, dtype=np.uint8)
+        # device state (allocated now, like the reference's f1/f2)
+        self.precision = precision
+        self._eng = _native.Engine((NI - 2, NJ - 2, NK - 2), rlx_scheme, kvisc, self._g,
+                                   self._bc, precision=precision, rho0=rho0, fp16_scale=fp16_scale,
+                                   layout=layout)
+
+        # flow field, isothermal.pyx:243-248, in pinned host memory that the
+        # engine reads and writes in place on every evolve call
+        self._phase = _native.host_array((NI, NJ, NK), np.uint8, SOLID)
         self.fluid_lsite_domain = None
-        self.den = np.ones((NI, NJ, NK), dtype=np.float64)
-        self.vel = np.zeros((NI, NJ, NK, 3), dtype=np.float64)
-        self.acc = np.zeros((NI, NJ, NK, 3), dtype=np.float64)
+        self._den = _native.host_array((NI, NJ, NK), np.float64, 1.0)
+        self._vel = _native.host_array((NI, NJ, NK, 3), np.float64, 0.0)
+        self._acc = _native.host_array((NI, NJ, NK, 3), np.float64, 0.0)
 
         # boundary condition data, isothermal.pyx:253-263
         self.bc_data_den_face_1X = np.ones((NJ, NK))
@@ -104,14 +112,7 @@ class Solver:
                        for l in range(LVECS)]
         self.tot_tsteps = 0
         self.init_done = 0
-
-        # device state (allocated now, like the reference's f1/f2)
-        self.precision = precision
-        self._eng = _native.Engine((NI - 2, NJ - 2, NK - 2), rlx_scheme, kvisc, self._g,
-                                   self._bc, precision=precision, rho0=rho0, fp16_scale=fp16_scale,
-                                   layout=layout)
         self._fluid = None
-        self._moving = False
 
     @classmethod
     def with_iflow(cls, phase, iden, ivel, rlx_scheme=BGK, kvisc=1.0 / 6.0, g=None, bc=None, **kw):
@@ -132,6 +133,22 @@ class Solver:
     # ------------------------------------------------------------------
     # read-only attributes of the reference (isothermal.pxd:136-176)
     # ------------------------------------------------------------------
+    @property
+    def phase(self):
+        return self._phase
+
+    @property
+    def den(self):
+        return self._den
+
+    @property
+    def vel(self):
+        return self._vel
+
+    @property
+    def acc(self):
+        return self._acc
+
     @property
     def bc(self):
         return list(self._bc)
@@ -295,29 +312,27 @@ class Solver:
         return a[1:self.NI - 1, 1:self.NJ - 1, 1:self.NK - 1]
 
     def _sync_inputs(self):
-        """Mutable inputs of evolve: gravity, acc field, Zou-He face data and,
-        with moving walls, the wall velocities and the densities that Ladd's
-        term reads in the first step (isothermal.pyx:1117-1119, 1293-1295)."""
+        """Mutable inputs of evolve that live outside the bound host fields:
+        gravity and the Zou-He face data (isothermal.pyx:476-502, 1293-1295).
+        The acceleration field, the wall velocities and the densities that
+        Ladd's term reads in the first step come from the host fields
+        themselves, which the engine reads at every evolve call."""
         e = self._eng
         e.set_gravity(self._g)
-        acc = self._interior(self.acc)
-        e.upload_acc(acc if np.any(acc) else None)
         for f in range(6):
             if self._bc[f] in _FIXED:
                 e.set_face_data(f, *self._face_arrays(f))
-        vi = self._interior(self.vel)
-        moving = bool(np.any(vi[~self._fluid]))
-        if moving or self._moving:
-            e.upload_den_vel(self._interior(self.den), vi)
-            e.refresh_walls()
-        self._moving = moving
 
-    def _pull_fields(self):
-        """den/vel of the fluid sites after the last device step."""
-        den, vel = self._eng.download_den_vel()
-        fl = self._fluid
-        self._interior(self.den)[fl] = den[fl]
-        self._interior(self.vel)[fl] = vel[fl]
+    def _load_inputs(self):
+        """_sync_inputs plus the host fields as the inputs of the next steps
+        (the set-up of an engine-level run such as the adjoint)."""
+        self._sync_inputs()
+        self._eng.load_host_fields()
+
+    def _pull_fields(self, halo_pass=True):
+        """den/vel of the fluid sites after the last device step into the host
+        fields, then (halo_pass) the den/vel halo pass."""
+        self._eng.store_host_fields(halo_pass)
 
     # ------------------------------------------------------------------
     # update methods
@@ -339,7 +354,7 @@ class Solver:
             n_dev = e.finalize()
             if n_dev != n:
                 raise RuntimeError(f'device fluid-site count {n_dev} != {n}')
-            self._moving = bool(np.any(self._interior(self.vel)[~self._fluid]))
+            e.bind_host_fields(self._phase, self._den, self._vel, self._acc)
         self.init_done = 1
         return n
 
@@ -357,16 +372,12 @@ class Solver:
         tsteps = int(tsteps) & 0xFFFFFFFF
         if tsteps > 0:
             self._sync_inputs()
-            e = self._eng
-            # The reference refreshes the halo den/vel from the interior before
-            # each step, so after the last step the halos hold the state of
-            # step T-1 (isothermal.pyx:1408-1415).
-            if tsteps > 1:
-                e.evolve(tsteps - 1)
-                self._pull_fields()
-            self._enforce_den_vel_per_sym_bc()
-            e.evolve(1)
-            self._pull_fields()
+            # On the device: the host fields are read as they are now (acc,
+            # wall velocities; a phase edit since finalize_init raises), the
+            # steps run, and den/vel of the fluid sites come back in place,
+            # with the halo den/vel of step T-1 as the reference leaves them
+            # (isothermal.pyx:1408-1415).
+            self._eng.evolve_host(tsteps)
         self.tot_tsteps = (self.tot_tsteps + tsteps) & 0xFFFFFFFF
         return self.tot_tsteps
 
@@ -411,11 +422,13 @@ class Solver:
         return out
 
     def flow_info(self):
-        """FP64 reductions on the GPU over the fluid sites: dict with total
-        mass, total velocity, total momentum and max |u| (the diagnostics of
-        the examples' loops and of calc_flow_info, solver.cpp:64-129)."""
-        r = self._eng.reduce()
-        return dict(tot_den=r[0], tot_vel=r[1:4].copy(), tot_mom=r[4:7].copy(), max_speed=r[7])
+        """calc_flow_info (include/solver/solver.cpp:64-129) on the GPU, in
+        FP64, over the fluid sites of the last step: dict with the fields of
+        IsothermalFlowInfo (include/solver/solver.h:13-15) -- tot_jx/jy/jz,
+        tot_j2 (sum den |u|), tot_den, ave_ux/uy/uz, ave_u2 (mean |u|), max_u2
+        (max |u|), max_den, min_den (extrema starting from rho0, as in the
+        reference) -- and fluid_sites."""
+        return self._eng.flow_info()
 
     def convergence(self):
         """Relative L2 change of the velocity field since the previous call

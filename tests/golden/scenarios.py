@@ -20,6 +20,10 @@ jyulb isothermal solver:
              isothermal.pyx:1465-1527 and the per-call _set_bc_data refresh)
   faces      optional Zou-He face data written into den/vel before
              finalize_init (inlet profile / outlet density), see build_inputs
+  den_profile  optional density per index along one axis (fluid sites)
+  vel_profile  optional list: one velocity component of a face layer per
+             index along one of the layer's axes
+  solid_layers, layer_vel  boundary layers made SOLID, and layer velocities
 
 The generator functions below produce the *interior* input arrays that the
 reference Solver receives through its aliasing views (get_phase/get_den/
@@ -83,6 +87,24 @@ def build_inputs(sc):
             prof = 1.0 + 0.3 * (rng.random(den[sl].shape) - 0.5) if fd.get("profile") else 1.0
             for c in range(3):
                 vel[sl + (c,)] = v[c] * prof
+    # per-layer profiles written through the views, as the example drivers do:
+    # a density per index along an axis (examples/poiseuille_pdiff.py:93-97)
+    # and one velocity component of a face layer per index along another axis
+    # (examples/poiseuille_invel.py:88-98)
+    if "den_profile" in sc:
+        dp = sc["den_profile"]
+        sh = [1, 1, 1]
+        sh[dp["axis"]] = -1
+        prof = np.broadcast_to(np.asarray(dp["values"], dtype=np.float64).reshape(sh), shape)
+        fl = phase != SOLID
+        den[fl] = prof[fl]
+    for vp in sc.get("vel_profile", []):
+        sl = _face_slices(vp["face"], shape)
+        layer = vel[sl + (vp["component"],)]
+        ax = [a for a in range(3) if a != vp["face"] // 2].index(vp["axis"])
+        sh = [1, 1]
+        sh[ax] = -1
+        layer[...] = np.asarray(vp["values"], dtype=np.float64).reshape(sh)
     return phase, den, vel, acc
 
 
@@ -179,6 +201,39 @@ def example_scenarios():
                     bc=[P] * 6, seed=0, calls=[500] * 20,
                     solid_layers=[(0, 1), (1, 1)],
                     layer_vel=[(0, 0.0, 0.0, 0.0), (1, 0.0, 1e-5, 2e-5)]))
+    # examples/poiseuille_pdiff.py:31-101 and poiseuille_pdiff_sym.py: flow in z
+    # driven by FIXED_DEN faces at 1Z/Z1, plates at 1Y (WALL) and Y1 (WALL, or
+    # SYMMETRIC for the half channel), x periodic; the initial density is the
+    # linear pressure profile, computed in the example's operation order
+    for name, H, rlx, kvisc, y1 in (("ex_poiseuille_pdiff", 11, BGK, 2.0 / 6.0, W),
+                                    ("ex_poiseuille_pdiff_sym", 6, TRT, 0.2 / 6.0, S)):
+        L, Wd = 10, 5
+        pgrad = 1.0 * -1e-5
+        pdiff = (L - 1.0) * pgrad
+        pin = (1.0 / 3.0) * 1.0 + (-0.5 * pdiff)
+        out.append(dict(name=name, lat=[Wd, H, L], rlx=rlx, kvisc=kvisc, g=[0.0, 0.0, 0.0],
+                        bc=[P, P, W, y1, FD, FD], seed=0, calls=[500] * 20,
+                        den_profile=dict(axis=Z, values=[3.0 * (pin + l * pgrad) for l in range(L)])))
+    # examples/poiseuille_invel.py:31-98 and poiseuille_invel_sym.py: FIXED_VEL
+    # inlet at 1Z carrying the parabola of peak 1e-3 (poise_plate_steady_state_umax
+    # over H, or over 2H for the half channel), FIXED_DEN outlet at Z1
+    for name, H, rlx, kvisc, y1, Hp in (("ex_poiseuille_invel", 11, BGK, 2.0 / 6.0, W, 11),
+                                        ("ex_poiseuille_invel_sym", 6, TRT, 0.2 / 6.0, S, 12)):
+        L, Wd, umax = 10, 5, 1e-3
+        ana = []
+        for x in range(Hp):
+            rel = ((0.5 + x) - 0.5 * Hp) / (0.5 * Hp)
+            ana.append(umax * (1.0 - rel * rel))
+        out.append(dict(name=name, lat=[Wd, H, L], rlx=rlx, kvisc=kvisc, g=[0.0, 0.0, 0.0],
+                        bc=[P, P, W, y1, FV, FD], seed=0, calls=[500] * 20,
+                        vel_profile=[dict(face=4, component=Z, axis=Y, values=ana[:H])]))
+    # examples/lid_driven_cavity.py:40-107: regularized, 102 x 1 x 102 with all
+    # faces periodic (PERIOD_W = 1 in y), solid layers at z = 0 and z = H+1
+    # (the lid, moving at ux = 0.05) and at x = 0 and x = 101; two of the
+    # example's 25 calls of 5000 steps
+    out.append(dict(name="ex_lid_driven_cavity", lat=[102, 1, 102], rlx=REG, kvisc=0.05 / 6.0,
+                    g=[0.0, 0.0, 0.0], bc=[P] * 6, seed=0, calls=[5000, 5000],
+                    solid_layers=[(4, 1), (5, 1), (0, 1), (1, 1)], layer_vel=[(5, 0.05, 0.0, 0.0)]))
     # Plain gravity-driven Poiseuille channel, D3Q19 BGK, FP64: 32x34x8, SOLID
     # rows j=0 and j=33, periodic everywhere, tau=0.8, gx=1e-6, 10000 steps
     out.append(dict(name="c1_poiseuille", lat=[32, 34, 8], rlx=BGK, kvisc=0.1, g=[1e-6, 0.0, 0.0],

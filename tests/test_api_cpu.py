@@ -81,8 +81,8 @@ def test_variant_names_checked_before_the_device():
     for kw in ({"precision": "fp24"}, {"layout": "dense"}, {"layout": 7}):
         with pytest.raises(ValueError, match="unknown"):
             Solver(lat, defs.TRT, 0.1, **kw)
-    assert {k for k in _native.LAYOUTS if isinstance(k, str)} == {"rows", "soa", "soa_padded", "sparse"}
-    assert [_native.LAYOUTS[k] for k in ("rows", "soa", "soa_padded", "sparse")] == [0, 1, 2, 3]
+    assert {k for k in _native.LAYOUTS if isinstance(k, str)} == {"rows", "soa", "soa_padded", "sparse", "auto"}
+    assert [_native.LAYOUTS[k] for k in ("rows", "soa", "soa_padded", "sparse", "auto")] == [0, 1, 2, 3, 4]
     if not torch.cuda.is_available():
         for kw in ({"layout": "sparse"}, {"precision": "fp32_storage", "layout": 3}):
             with pytest.raises(RuntimeError, match="no HIP device"):
@@ -107,30 +107,84 @@ def test_flow_field_closed_forms():
 
 def test_package_exports():
     assert jyulb.__version__ == "0.2.0"
-    from jyulb import adjoint, bcs  # noqa: F401
+    from jyulb import adjoint  # noqa: F401
     assert callable(adjoint.flux_objective)
     assert callable(adjoint.flux_gradient)
 
 
 def test_checkpoint_interval_choice():
-    """interval='auto' of the gradient entry points: the largest interval whose
-    checkpoints and (rho, u*) record fit the budget. Lattice 10 x 4 x 70 has
+    """interval='auto' of the gradient entry points: the fewest checkpoint
+    segments whose memory fits the budget. Lattice 10 x 4 x 70 has
     10 * 4 * 128 = 5120 padded sites: a population array is 778,240 bytes in
-    FP64 and one step of record 163,840."""
+    FP64, one step of (rho, u*) record 163,840, and the reverse pass holds a
+    cotangent array and 3 acceleration cotangents per site (901,120 bytes)."""
     from types import SimpleNamespace
     from jyulb.adjoint import checkpoint_interval
     from jyulb._native import LB_FP64, LB_FP32_SHIFTED
     s64 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64))
     s32 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP32_SHIFTED))
-    # one segment: 2 arrays + 100 steps of record = 17,940,480 bytes
-    assert checkpoint_interval(s64, 100, 17_940_480) == 100
-    # a byte less: 2 segments need 3 arrays (2,334,720) + i steps, i <= 95
-    assert checkpoint_interval(s64, 100, 17_940_479) == 95
+    fb, rb, extra = 778_240, 163_840, 901_120
+    one = 2 * fb + 100 * rb + extra  # one segment: 18,841,600 bytes
+    assert checkpoint_interval(s64, 100, one) == 100
+    # a byte less: two segments of 50 (3 arrays + 50 steps of record), not the
+    # largest interval that fits (which would recompute more)
+    assert checkpoint_interval(s64, 100, one - 1) == 50
+    assert checkpoint_interval(s64, 100, 3 * fb + 50 * rb + extra) == 50
+    assert checkpoint_interval(s64, 100, 3 * fb + 50 * rb + extra - 1) == 34
     # FP32 halves every figure
-    assert checkpoint_interval(s32, 100, 17_940_480 // 2) == 100
-    assert checkpoint_interval(s32, 100, 17_940_480 // 2 - 1) == 95
-    # too small for any interval (the least footprint is 20 segments: 6 arrays + 20 steps)
+    assert checkpoint_interval(s32, 100, one // 2) == 100
+    assert checkpoint_interval(s32, 100, one // 2 - 1) == 50
+    # the dissipation objective also holds 23 seed values per site
+    assert checkpoint_interval(s64, 100, one, objective="dissipation") == 50
+    assert checkpoint_interval(s64, 100, one + 23 * 5120 * 8, objective="dissipation") == 100
+    # the least footprint is 5 segments of 20 (6 arrays + 20 steps): 8,847,360 bytes
+    assert checkpoint_interval(s64, 100, 6 * fb + 20 * rb + extra) == 20
     with pytest.raises(MemoryError):
-        checkpoint_interval(s64, 100, 6 * 778_240 + 17 * 163_840)
+        checkpoint_interval(s64, 100, 6 * fb + 20 * rb + extra - 1)
     with pytest.raises(ValueError):
         checkpoint_interval(s64, 0, 1 << 40)
+
+
+@pytest.mark.parametrize("lat", [(1024, 1024, 2048), (1024, 1024, 2049), (4096, 1024, 1024)])
+def test_lattice_limit_is_checked_before_the_device(lat):
+    """Sites are indexed with 32-bit integers and counted with ints: lb_create
+    refuses nx * ny * round_up(nz, 64) >= 2^31 with one message, before it
+    touches a device (so this runs on the CPU). 1024 x 1024 x 2048 is exactly
+    2^31 padded cells; 2049 pads to 2112."""
+    import ctypes
+    from jyulb import _native
+    L = _native.lib()
+    cfg = _native.LbConfig()
+    for a in range(3):
+        cfg.lat[a] = lat[a]
+    cfg.rlx_scheme, cfg.kvisc, cfg.precision, cfg.layout = 0, 0.1, 2, 0
+    for f in range(6):
+        cfg.bc[f] = 1
+    h = ctypes.c_void_p()
+    rc = L.lb_create(ctypes.byref(cfg), ctypes.byref(h))
+    msg = L.lb_last_error().decode()
+    padded = lat[0] * lat[1] * ((lat[2] + 63) // 64 * 64)
+    assert padded >= 2 ** 31
+    assert rc == _native.LB_EINVAL
+    assert msg == (f"lattice of {padded} cells (z padded to a multiple of 64) exceeds the engine's "
+                   "limit of 2^31 padded cells")
+
+
+def test_lattice_below_the_limit_passes_the_check():
+    """One row fewer than 2^31 padded cells passes the size check: without a
+    device lb_create then fails for want of a GPU, not for the size."""
+    import ctypes
+    import torch
+    from jyulb import _native
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: lb_create would allocate 2^31 sites")
+    L = _native.lib()
+    cfg = _native.LbConfig()
+    for a, n in enumerate((1023, 1024, 2048)):
+        cfg.lat[a] = n
+    cfg.rlx_scheme, cfg.kvisc, cfg.precision, cfg.layout = 0, 0.1, 2, 0
+    for f in range(6):
+        cfg.bc[f] = 1
+    h = ctypes.c_void_p()
+    assert L.lb_create(ctypes.byref(cfg), ctypes.byref(h)) == _native.LB_EHIP
+    assert "no HIP device" in L.lb_last_error().decode()

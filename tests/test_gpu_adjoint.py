@@ -153,9 +153,10 @@ def test_adjoint_periodic_and_checkpoint_invariance():
 def test_adjoint_forward_record_reused_and_invalidated(layout, obj):
     """The forward pass records (rho, u*) over its last segment and the reverse
     pass skips that segment's recompute. A second reverse pass (interval <
-    steps: the record was overwritten by an earlier segment) and one after an
-    input is set again (the record is dropped) recompute instead; all agree
-    bit for bit, and with interval >= steps nothing is recomputed."""
+    steps: the record was overwritten by an earlier segment) recomputes
+    instead; both agree bit for bit, and with interval >= steps nothing is
+    recomputed. An input set between the passes refuses the reverse pass
+    (its recompute would run on other inputs than the forward pass's)."""
     s, prof, amp = make_channel(TRT, "fp64", seed=6, layout=layout)
     acc, fden, fvel, g = _inputs(s)
     eng = s.engine
@@ -165,11 +166,11 @@ def test_adjoint_forward_record_reused_and_invalidated(layout, obj):
         print(f"adjoint record {layout} obj {obj} interval {interval}", flush=True)
         J, ga, gf = _adjoint(s, acc, fden, fvel, g, 29, interval, obj)
         again = eng.adj_backward(FACE_1X)
+        np.testing.assert_array_equal(again[0], ga)
+        np.testing.assert_array_equal(again[1], gf)
         eng.set_face_data(FACE_1X, fden, fvel)
-        fresh = eng.adj_backward(FACE_1X)
-        for a, b in (again, fresh):
-            np.testing.assert_array_equal(a, ga)
-            np.testing.assert_array_equal(b, gf)
+        with pytest.raises(RuntimeError, match="needs the lb_adj_forward pass"):
+            eng.adj_backward(FACE_1X)
         if ref is None:
             ref = (J, ga, gf)
         assert J == ref[0]
@@ -212,8 +213,52 @@ def test_adjoint_failed_forward_leaves_no_reverse_pass():
     assert np.max(np.abs(ga)) > 0
     with pytest.raises(ValueError, match="no fluid site"):
         eng.adj_forward(6, 4, FACE_X1, X)
-    with pytest.raises(RuntimeError, match="must precede"):
+    with pytest.raises(RuntimeError, match="needs the lb_adj_forward pass"):
         eng.adj_backward()
+
+
+def test_adjoint_reverse_pass_tied_to_its_forward_pass():
+    """An evolve call after the forward pass moves the state on: the reverse
+    pass is refused. A reverse pass right after its forward pass leaves the
+    populations, the step count and the AA parity as the forward pass left
+    them, so evolving on gives the same bits as evolving throughout."""
+    s, _, _ = make_channel(TRT, "fp64", seed=12)
+    t, _, _ = make_channel(TRT, "fp64", seed=12)
+    eng = s.engine
+    s._set_bc_data()
+    s._load_inputs()
+    t._set_bc_data()
+    t._load_inputs()
+    eng.adj_forward(7, 3, FACE_X1, X)
+    eng.evolve(3)
+    with pytest.raises(RuntimeError, match="needs the lb_adj_forward pass"):
+        eng.adj_backward(FACE_1X)
+    eng.adj_forward(7, 3, FACE_X1, X)  # from step 10
+    ga, gf = eng.adj_backward(FACE_1X)
+    assert np.max(np.abs(ga)) > 0
+    assert eng.info()["steps"] == 17 and eng.info()["next_odd"] == 0
+    eng.evolve(5)
+    t.engine.evolve(22)
+    s._pull_fields()
+    t._pull_fields()
+    np.testing.assert_array_equal(np.asarray(s.get_den()), np.asarray(t.get_den()))
+    np.testing.assert_array_equal(np.asarray(s.get_vel()), np.asarray(t.get_vel()))
+
+
+def test_autograd_backward_refused_after_another_forward():
+    """Two objectives evaluated on one solver: the first one's backward would
+    run against the second forward pass's checkpoints, so it raises."""
+    import torch
+    from jyulb.adjoint import flux_objective
+    s, prof, amp = make_channel(BGK, "fp64", seed=13)
+    acc = torch.tensor(np.ascontiguousarray(s.get_acc()), device="cuda", requires_grad=True)
+    J1 = flux_objective(acc, None, s, 12, 4, FACE_X1, X)
+    J2 = flux_objective(acc, None, s, 12, 4, FACE_X1, X)
+    with pytest.raises(RuntimeError, match="another adjoint forward pass"):
+        (J1 + J2).backward()
+    J3 = flux_objective(acc, None, s, 12, 4, FACE_X1, X)
+    J3.backward()  # its own forward pass: fine
+    assert acc.grad is not None and float(acc.grad.abs().max()) > 0
 
 
 def test_adjoint_torch_autograd_function():

@@ -115,11 +115,14 @@ def test_duct_full_size_precision():
 def test_fp64_reductions_and_convergence(precision):
     den, vel, fl, info, s = duct_fields(precision)
     d, v = den[fl], vel[fl]
+    n = d.size
+    assert info["fluid_sites"] == n
     assert abs(info["tot_den"] - d.sum()) <= 1e-13 * d.sum()
-    np.testing.assert_allclose(info["tot_vel"], v.sum(0), rtol=1e-12, atol=1e-13 * np.abs(v).sum())
-    np.testing.assert_allclose(info["tot_mom"], (d[:, None] * v).sum(0), rtol=1e-12,
-                               atol=1e-13 * np.abs(v).sum())
-    assert info["max_speed"] == pytest.approx(np.sqrt((v * v).sum(1)).max(), rel=1e-15)
+    np.testing.assert_allclose([info["ave_ux"], info["ave_uy"], info["ave_uz"]], v.sum(0) / n, rtol=1e-12,
+                               atol=1e-13 * np.abs(v).sum() / n)
+    np.testing.assert_allclose([info["tot_jx"], info["tot_jy"], info["tot_jz"]], (d[:, None] * v).sum(0),
+                               rtol=1e-12, atol=1e-13 * np.abs(v).sum())
+    assert info["max_u2"] == pytest.approx(np.sqrt((v * v).sum(1)).max(), rel=1e-15)
     # steady-state test: relative L2 change of u between two calls
     assert s.convergence() == 1.0
     s.evolve(25)

@@ -41,7 +41,7 @@ def _run_fixture(name, **kw):
 @pytest.mark.parametrize("name", NAMES)
 def test_sparse_equals_rows_bitwise(name, precision):
     print(f"scenario {name} precision {precision}", flush=True)
-    a = _run_fixture(name, precision=precision)
+    a = _run_fixture(name, precision=precision, layout="rows")
     b = _run_fixture(name, precision=precision, layout="sparse")
     assert a.keys() == b.keys()
     for k in a:
@@ -76,7 +76,7 @@ def test_sparse_porous_medium_equals_rows(precision):
     from jyulb.isothermal import Solver
     lat, steps = (72, 40, 90), (7, 40, 53)
     print(f"porous {lat} {precision}", flush=True)
-    a, ph = _porous_run(Solver, lat, steps, precision=precision)
+    a, ph = _porous_run(Solver, lat, steps, precision=precision, layout="rows")
     b, _ = _porous_run(Solver, lat, steps, precision=precision, layout="sparse")
     assert a[0] == b[0] == int((ph != 0).sum())
     for x, y in zip(a[1:], b[1:]):
@@ -135,7 +135,7 @@ def test_sparse_device_memory_follows_fluid_sites():
     lat = (64, 32, 64)
     a, ph = _porous_run(Solver, lat, (2,))
     nf = a[0]
-    rows = Solver(np.array(lat, dtype=np.int32), 0, 0.1)
+    rows = Solver(np.array(lat, dtype=np.int32), 0, 0.1, layout="rows")
     rows.get_phase()[...] = ph
     rows.finalize_init()
     sp = Solver(np.array(lat, dtype=np.int32), 0, 0.1, layout="sparse")

@@ -50,7 +50,7 @@ LBD void zouhe_site(const StepParams& p, uint32_t m, int i, int j, int k, TC rho
         else if (fc < 4) fi = (int64_t)(i + 1) * p.NK + (k + 1);
         else fi = (int64_t)(i + 1) * p.NJ + (j + 1);
         TC vb[3] = {(TC)p.fvel[fc][3 * fi], (TC)p.fvel[fc][3 * fi + 1], (TC)p.fvel[fc][3 * fi + 2]};
-        if (p.d.bc[fc] == FIXED_DEN) zh_fix_den<TC, SH>(fc, (TC)p.fden[fc][fi], vb, rho0, f);
+        if (p.d.bc[fc] == FIXED_DEN) zh_fix_den<TC, SH>(fc, (TC)p.fden[fc][fi], (TC)(p.fden[fc][fi] - p.rho0), vb, f);
         else zh_fix_vel<TC, SH>(fc, vb, rho0, f);
     }
 }
@@ -200,15 +200,18 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     }
     TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
     TC vel[3];
-    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel);
+    TC ds;
+    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel, ds);
 
     if (p.store_dv) {
-        p.den[c] = (double)den;
+        // shifted variants: rho0 + (den - rho0) in FP64, so the stored density
+        // keeps the deviation's precision (it feeds FIXED_DEN face data back)
+        p.den[c] = SH ? p.rho0 + (double)ds : (double)den;
         p.vel[c] = (double)vel[0];
         p.vel[nq + c] = (double)vel[1];
         p.vel[2 * nq + c] = (double)vel[2];
     } else if (MODE != MODE_PLAIN && p.moving && (m & M_MOVING)) {
-        p.den[c] = (double)den;
+        p.den[c] = SH ? p.rho0 + (double)ds : (double)den;
     }
     if (p.rec) {
         TC* R = (TC*)p.rec;
@@ -420,14 +423,15 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
     }
     TC prm[2] = {(TC)p.prm[0], (TC)p.prm[1]};
     TC vel[3];
-    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel);
+    TC ds;
+    const TC den = collide<RLX, TC, SH, P == LB_FP32_SHIFTED>(f, acc, prm, rho0, vel, ds);
     if (p.store_dv) {
-        p.den[c] = (double)den;
+        p.den[c] = SH ? p.rho0 + (double)ds : (double)den;
         p.vel[c] = (double)vel[0];
         p.vel[nq + c] = (double)vel[1];
         p.vel[2 * nq + c] = (double)vel[2];
     } else if (FULL && p.moving && (m & M_MOVING)) {
-        p.den[c] = (double)den;
+        p.den[c] = SH ? p.rho0 + (double)ds : (double)den;
     }
     if (p.rec) {
         TC* R = (TC*)p.rec;

@@ -155,12 +155,15 @@ LBD void forcing(T* h, const T* acc, T den, T jx, T jy, T jz, T c1, T c2) {
 }
 
 // forw_euler_BGK, D3Q19.pyx:655-740. prm[0] = 1/tau. Returns den; vel gets
-// the half-force shifted velocity.
+// the half-force shifted velocity, ds the density as the populations hold it
+// (den, or den - rho0 when SH: the caller can add rho0 back in FP64, where
+// den itself carries only the compute type's precision near rho0).
 template <class T, bool SH, bool DIV = false>
-LBD T collide_bgk(T* f, const T* acc, const T* prm, T rho0, T* vel) {
+LBD T collide_bgk(T* f, const T* acc, const T* prm, T rho0, T* vel, T& ds) {
     T inv_tau = prm[0], rlx_cff1 = T(1.0) - inv_tau, rlx_cff2 = (T(1.0) - T(0.5) * inv_tau);
     T h[10], den;
     T s = moments_den_vel<T, SH>(f, rho0, den, vel);
+    ds = s;
     vel[0] += T(0.5) * acc[0];
     vel[1] += T(0.5) * acc[1];
     vel[2] += T(0.5) * acc[2];
@@ -178,7 +181,7 @@ LBD T collide_bgk(T* f, const T* acc, const T* prm, T rho0, T* vel) {
 
 // forw_euler_TRT, D3Q19.pyx:745-853. prm[0] = 1/tau_even, prm[1] = 1/tau_odd.
 template <class T, bool SH, bool DIV = false>
-LBD T collide_trt(T* f, const T* acc, const T* prm, T rho0, T* vel) {
+LBD T collide_trt(T* f, const T* acc, const T* prm, T rho0, T* vel, T& ds) {
     T inv_tau_e = prm[0], inv_tau_o = prm[1];
     T rlx_cff1 = T(1.0) - T(0.5) * inv_tau_o;
     T rlx_cff2 = T(1.0) - T(0.5) * inv_tau_e;
@@ -186,6 +189,7 @@ LBD T collide_trt(T* f, const T* acc, const T* prm, T rho0, T* vel) {
     T rlx_cff4 = T(0.5) * (inv_tau_e - inv_tau_o);
     T h[10], den;
     T s = moments_den_vel<T, SH>(f, rho0, den, vel);
+    ds = s;
     vel[0] += T(0.5) * acc[0];
     vel[1] += T(0.5) * acc[1];
     vel[2] += T(0.5) * acc[2];
@@ -212,10 +216,11 @@ LBD T collide_trt(T* f, const T* acc, const T* prm, T rho0, T* vel) {
 
 // forw_euler_regul, D3Q19.pyx:859-940
 template <class T, bool SH, bool DIV = false>
-LBD T collide_reg(T* f, const T* acc, const T* prm, T rho0, T* vel) {
+LBD T collide_reg(T* f, const T* acc, const T* prm, T rho0, T* vel, T& ds) {
     T inv_tau = prm[0], rlx_cff1 = T(1.0) - inv_tau, rlx_cff2 = (T(1.0) - T(0.5) * inv_tau);
     T he[10], h[10];
     moments_h<T>(f, h);
+    ds = h[0];
     T den = SH ? rho0 + h[0] : h[0];
     T inv_den = T(1.0) / den;
     T jx = h[1] + T(0.5) * acc[0] * den;
@@ -239,10 +244,10 @@ LBD T collide_reg(T* f, const T* acc, const T* prm, T rho0, T* vel) {
 // its error against FP64 on a 512x128x128 duct drops from 2.4e-5 to 1.2e-6;
 // FP16 storage, whose error is its storage, keeps the plain products)
 template <int RLX, class T, bool SH, bool DIV = false>
-LBD T collide(T* f, const T* acc, const T* prm, T rho0, T* vel) {
-    if constexpr (RLX == BGK) return collide_bgk<T, SH, DIV>(f, acc, prm, rho0, vel);
-    else if constexpr (RLX == TRT) return collide_trt<T, SH, DIV>(f, acc, prm, rho0, vel);
-    else return collide_reg<T, SH, DIV>(f, acc, prm, rho0, vel);
+LBD T collide(T* f, const T* acc, const T* prm, T rho0, T* vel, T& ds) {
+    if constexpr (RLX == BGK) return collide_bgk<T, SH, DIV>(f, acc, prm, rho0, vel, ds);
+    else if constexpr (RLX == TRT) return collide_trt<T, SH, DIV>(f, acc, prm, rho0, vel, ds);
+    else return collide_reg<T, SH, DIV>(f, acc, prm, rho0, vel, ds);
 }
 
 // ---- Zou-He faces ----------------------------------------------------------
@@ -332,13 +337,15 @@ LBD void zh_sums(int face, const T* f, T& den_kn, T& jout) {
 }
 
 // fix_den_face_*, bcs.pyx:31-260: the density and the tangential velocity are
-// enforced, the normal momentum follows from mass balance.
+// enforced, the normal momentum follows from mass balance. `ddev` is den -
+// rho0, formed by the caller before rounding to T (the shifted sums miss
+// rho0 * (5/6) and rho0 * (1/6), which it restores; rounding den itself to
+// FP32 would cost the deviation up to 6e-8); unused without SH.
 template <class T, bool SH>
-LBD void zh_fix_den(int face, T den, const T* vel, T rho0, T* f) {
+LBD void zh_fix_den(int face, T den, T ddev, const T* vel, T* f) {
     T den_kn, jout;
     zh_sums<T>(face, f, den_kn, jout);
-    // shifted sums miss rho0 * (5/6) and rho0 * (1/6): den - rho0 restores them
-    T dd = SH ? den - rho0 : den;
+    T dd = SH ? ddev : den;
     int axis = face >> 1;
     bool low = (face & 1) == 0;
     T jn;

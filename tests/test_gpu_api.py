@@ -177,18 +177,32 @@ def test_fp16_guard_gravity_channel_from_rest():
 
 
 def test_fp16_guard_zou_he_inlet_from_rest():
-    """A Zou-He inlet of 0.05 switched on for a fluid at rest: the scale is
-    sized from the face data at finalize_init and rechecked when the face
-    data change between calls, so no population overflows."""
-    import bench
-    s16, _ = bench.duct("fp16", lat=(48, 12, 12), peak=0.0)
-    s64, _ = bench.duct("fp64", lat=(48, 12, 12), peak=0.0)
-    for s in (s16, s64):
+    """A Zou-He inlet of 0.05 switched on for a fluid at rest: the scale chosen
+    at finalize_init for the fluid at rest (2^23) would overflow on the first
+    step (|f - w rho0| ~ 3e-2 at the inlet). The guard sizes it again from the
+    face data at the start of the call that sees them, by an exact power of
+    two (2^-9), so the run is bit for bit the run with that scale (2^14) from
+    the start, and finite. Against FP64 this impulsive start keeps the FP16
+    storage error of its acoustic transient (rel. L2 1.7e-2 measured with
+    either scale)."""
+    from jyulb.isothermal import Solver
+    from jyulb.defs import REGULARIZATION
+
+    def run(prec, **kw):
+        s = Solver(np.array([48, 12, 12], dtype=np.int32), REGULARIZATION, (0.55 - 0.5) / 3.0, np.zeros(3),
+                   np.array([FIXED_VEL, FIXED_DEN, WALL, WALL, WALL, WALL], dtype=np.int32), precision=prec, **kw)
+        s.get_phase()[...] = 1
+        s.finalize_init()
         s.evolve(10)
         s.get_vel()[0, :, :, 0] = 0.05  # the inlet layer: _set_bc_data copies it to the face data
         s.evolve(300)
-    v16, v64 = np.asarray(s16.get_vel()), np.asarray(s64.get_vel())
+        return s
+
+    auto, fixed, s64 = run("fp16"), run("fp16", fp16_scale=2.0 ** 14), run("fp64")
+    assert auto.engine.info()["fp16_scale"] == 2.0 ** 14
+    v16, vfix, v64 = (np.asarray(x.get_vel()) for x in (auto, fixed, s64))
     assert np.isfinite(v16).all()
-    err = float(np.max(np.abs(v16 - v64))) / float(np.max(np.abs(v64)))
-    print(f"FP16 inlet: scale {s16.engine.info()['fp16_scale']}, max rel velocity error {err:.3e}", flush=True)
-    assert err <= 2e-2
+    np.testing.assert_array_equal(v16, vfix)
+    l2 = float(np.sqrt(np.sum((v16 - v64) ** 2) / np.sum(v64 ** 2)))
+    print(f"FP16 inlet: rel. L2 velocity error against FP64 {l2:.3e}", flush=True)
+    assert l2 <= 3e-2

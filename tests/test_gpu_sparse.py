@@ -154,11 +154,12 @@ def test_sparse_device_memory_follows_fluid_sites():
 
 @pytest.mark.parametrize("layout", ["rows", "sparse"])
 def test_lattice_beyond_32_bit_site_index_is_refused(layout):
-    """Sites are indexed with 32 bits; a lattice of 2^32 or more padded cells
-    is refused by lb_create before it allocates anything (the engine handle
-    is created directly: the Solver would first build its host arrays)."""
+    """Sites are indexed with 32-bit integers and counted with ints; a lattice
+    of 2^31 or more padded cells (here 2^32) is refused by lb_create before
+    it allocates anything (the engine handle is created directly: the Solver
+    would first build its host arrays)."""
     from jyulb import _native
     from jyulb.defs import BGK, PERIODIC
-    with pytest.raises(ValueError, match="2\\^32"):
+    with pytest.raises(ValueError, match="limit of 2\\^31 padded cells"):
         _native.Engine((4096, 1024, 1024), BGK, 0.1, np.zeros(3), [PERIODIC] * 6, precision="fp16",
                        layout=layout)

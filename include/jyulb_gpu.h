@@ -188,6 +188,22 @@ typedef struct {
 } lb_flow_info_t;
 int lb_flow_info(lb_solver *s, lb_flow_info_t *info);
 
+/* No counterpart in the reference. The calc_flow_info of lb_flow_info for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops): the fields are caller-provided device arrays in the
+ * interior C-order layout (phase u8 [nx][ny][nz], 0 = SOLID; den f64
+ * [nx][ny][nz]; vel f64 [nx][ny][nz][3]), the result is written to the
+ * device array info_dev (13 doubles in lb_flow_info_t order), and the
+ * per-block partials live in a fixed 104 KiB array of the library on each
+ * device, so the call allocates nothing; it launches two kernels on
+ * hip_stream (NULL: the default stream) and returns without synchronising.
+ * At most 2^31 - 1 sites (nx * ny * nz); larger lattices are refused with
+ * LB_EINVAL. Calls that overlap on one device share that array and race:
+ * order them on one stream. The first call on a device resolves the array's
+ * address there (make it outside graph capture). */
+int lb_flow_info_fields_noalloc(const uint8_t *phase_dev, const double *den_dev, const double *vel_dev,
+                                const int32_t lat[3], double rho0, double *info_dev, void *hip_stream);
+
 /* Steady-state test in FP64: ||u - u_prev||_2 / ||u||_2 over the fluid
  * sites, then u_prev := u. The first call returns 1.0. */
 int lb_convergence(lb_solver *s, double *rel_change);

@@ -35,6 +35,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <mutex>
 #include <string>
 
 #include "lb_common.h"
@@ -405,6 +406,8 @@ __global__ __launch_bounds__(RED_BLOCK) void k_flow1(S sites, double rho0, doubl
     }
 }
 
+// Final reduction of the per-block partials, written as lb_flow_info_t
+// (IsothermalFlowInfo, solver.cpp:112-128, plus the fluid-site count).
 __global__ __launch_bounds__(RED_BLOCK) void k_flow2(const double* part, int nparts, double rho0, double* out) {
     double a[FLOW_N];
 #pragma unroll
@@ -413,6 +416,7 @@ __global__ __launch_bounds__(RED_BLOCK) void k_flow2(const double* part, int npa
 #pragma unroll
         for (int q = 0; q < FLOW_N; ++q) a[q] = flow_op(q, a[q], part[(int64_t)b * FLOW_N + q]);
     __shared__ double sh[RED_BLOCK / 64][FLOW_N];
+    __shared__ double r[FLOW_N];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < FLOW_N; ++q) {
@@ -425,28 +429,27 @@ __global__ __launch_bounds__(RED_BLOCK) void k_flow2(const double* part, int npa
         const int q = threadIdx.x;
         double s = sh[0][q];
         for (int w2 = 1; w2 < RED_BLOCK / 64; ++w2) s = flow_op(q, s, sh[w2][q]);
-        out[q] = s;
+        r[q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double n = r[FI_CNT];
+        const double ave_cff = n > 0 ? 1.0 / n : 0.0;
+        const double v[FLOW_N] = {r[FI_JX], r[FI_JY], r[FI_JZ], r[FI_J2], r[FI_DEN], ave_cff * r[FI_UX],
+                                  ave_cff * r[FI_UY], ave_cff * r[FI_UZ], ave_cff * r[FI_U2], r[FI_MAXU],
+                                  r[FI_MAXDEN], r[FI_MINDEN], n};
+        for (int q = 0; q < FLOW_N; ++q) out[q] = v[q];
     }
 }
+static_assert(sizeof(lb_flow_info_t) == FLOW_N * sizeof(double), "lb_flow_info_t is FLOW_N doubles");
 
-// IsothermalFlowInfo from the reduced slots (solver.cpp:112-128)
-static void flow_info_from(const double* r, lb_flow_info_t* info) {
-    info->tot_jx = r[FI_JX];
-    info->tot_jy = r[FI_JY];
-    info->tot_jz = r[FI_JZ];
-    info->tot_j2 = r[FI_J2];
-    info->tot_den = r[FI_DEN];
-    const double n = r[FI_CNT];
-    const double ave_cff = n > 0 ? 1.0 / n : 0.0;
-    info->ave_ux = ave_cff * r[FI_UX];
-    info->ave_uy = ave_cff * r[FI_UY];
-    info->ave_uz = ave_cff * r[FI_UZ];
-    info->ave_u2 = ave_cff * r[FI_U2];
-    info->max_u2 = r[FI_MAXU];
-    info->max_den = r[FI_MAXDEN];
-    info->min_den = r[FI_MINDEN];
-    info->fluid_sites = n;
-}
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free entry point lb_flow_info_fields_noalloc: the
+// per-block partials of k_flow1 (RED_PARTS x FLOW_N doubles, 104 KiB per
+// device). In host code the name is only a stand-in for the symbol; its
+// address on each device comes from hipGetSymbolAddress (scratch_ptr).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[1024 * FLOW_N];
 
 // ---------------------------------------------------------------------------
 // the Solver's halo-padded host fields (lb_bind_host_fields / lb_evolve_host),
@@ -686,6 +689,7 @@ struct lb_solver {
 };
 
 static constexpr int RED_PARTS = 1024;
+static_assert(sizeof(scratch) == sizeof(double) * RED_PARTS * FLOW_N, "scratch holds RED_PARTS partials");
 
 static size_t ts_size(int prec) { return prec == LB_FP64 ? 8 : prec == LB_FP16_STORAGE ? 2 : 4; }
 static size_t tc_size(int prec) { return prec == LB_FP64 || prec == LB_FP32_STORAGE ? 8 : 4; }
@@ -1866,10 +1870,54 @@ int lb_flow_info(lb_solver* s, lb_flow_info_t* info) {
     k_flow1<SolverSites><<<nb, RED_BLOCK, 0, s->stream>>>(sites, s->rho0, s->part);
     k_flow2<<<1, RED_BLOCK, 0, s->stream>>>(s->part, nb, s->rho0, s->red);
     HIPCHK(hipGetLastError());
-    double r[FLOW_N];
-    HIPCHK(hipMemcpyAsync(r, s->red, FLOW_N * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(info, s->red, sizeof(lb_flow_info_t), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    flow_info_from(r, info);
+    return LB_OK;
+}
+
+// Address of `scratch` on the current device, resolved once per device with
+// that device current (each device holds its own copy of the array).
+static double* scratch_ptr(int* rc_out) {
+    constexpr int MAX_DEV = 64;
+    static double* ptrs[MAX_DEV] = {};
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= MAX_DEV) {
+        *rc_out = fail(LB_EHIP, "hipGetDevice failed or device %d beyond %d: %s", dev, MAX_DEV, hipGetErrorString(e));
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *rc_out = fail(LB_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        ptrs[dev] = (double*)p;
+    }
+    *rc_out = LB_OK;
+    return ptrs[dev];
+}
+
+int lb_flow_info_fields_noalloc(const uint8_t* phase_dev, const double* den_dev, const double* vel_dev,
+                                const int32_t lat[3], double rho0, double* info_dev, void* hip_stream) {
+    if (!phase_dev || !den_dev || !vel_dev || !lat || !info_dev) return fail(LB_EINVAL, "null argument");
+    if (lat[0] < 1 || lat[1] < 1 || lat[2] < 1) return fail(LB_EINVAL, "Invalid lattice size input.");
+    const uint64_t n = (uint64_t)lat[0] * (uint64_t)lat[1] * (uint64_t)lat[2];
+    if (n >= (1ull << 31))
+        return fail(LB_EINVAL, "lb_flow_info_fields_noalloc: %llu sites exceed its limit of 2^31 - 1",
+                    (unsigned long long)n);
+    int rc;
+    double* part = scratch_ptr(&rc);
+    if (!part) return rc;
+    const hipStream_t st = (hipStream_t)hip_stream;
+    const int nb = (int)std::min<uint64_t>(RED_PARTS, (n + RED_BLOCK - 1) / RED_BLOCK);
+    PlainSites sites{phase_dev, den_dev, vel_dev, (uint32_t)n};
+    k_flow1<PlainSites><<<nb, RED_BLOCK, 0, st>>>(sites, rho0, part);
+    k_flow2<<<1, RED_BLOCK, 0, st>>>(part, nb, rho0, info_dev);
+    HIPCHK(hipGetLastError());
     return LB_OK;
 }
 

@@ -108,6 +108,9 @@ def lib():
         "lb_evolve_host": ([H, ctypes.c_uint32], ctypes.c_int),
         "lb_store_host_fields": ([H, ctypes.c_int], ctypes.c_int),
         "lb_layout": ([H, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "lb_flow_info_fields_noalloc": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_double, ctypes.c_void_p,
+                                         ctypes.c_void_p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -254,6 +257,10 @@ class Engine:
         vel = np.empty((nx, ny, nz, 3))
         self.call("lb_download_den_vel", ptr(den), ptr(vel))
         return den, vel
+
+    def download_den_vel_device(self, den_ptr, vel_ptr):
+        """den/vel into device arrays (interior C order), on the handle's stream."""
+        self.call("lb_download_den_vel_device", ctypes.c_void_p(den_ptr), ctypes.c_void_p(vel_ptr))
 
     def download_post_f(self):
         nx, ny, nz = self.lat

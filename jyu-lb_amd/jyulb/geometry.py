@@ -26,6 +26,9 @@ def porous_spheres(lat, rmin, rmax, solid_fraction, seed, open_faces=(), open_de
         sl[f // 2] = slice(0, open_depth) if f % 2 == 0 else slice(max(n - open_depth, 0), n)
         keep[tuple(sl)] = False
     target = solid_fraction * solid.size
+    if target > int(keep.sum()):
+        raise ValueError(f"solid fraction {solid_fraction} needs {int(np.ceil(target))} solid cells; only "
+                         f"{int(keep.sum())} lie outside the open boundary layers")
     count = 0
     while count < target:
         c = rng.random(3) * np.array(lat)

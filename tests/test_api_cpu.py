@@ -188,3 +188,13 @@ def test_lattice_below_the_limit_passes_the_check():
     h = ctypes.c_void_p()
     assert L.lb_create(ctypes.byref(cfg), ctypes.byref(h)) == _native.LB_EHIP
     assert "no HIP device" in L.lb_last_error().decode()
+
+
+def test_porous_spheres_refuses_an_impossible_solid_fraction():
+    """Open boundary layers that leave too few cells for the requested solid
+    fraction raise instead of looping forever."""
+    from jyulb.geometry import porous_spheres
+    with pytest.raises(ValueError, match="open boundary layers"):
+        porous_spheres((12, 10, 9), 4, 8, 0.10, 4, open_faces=(0, 1), open_depth=8)
+    s = porous_spheres((20, 12, 10), 2, 4, 0.20, 5)
+    assert 0.20 <= s.mean() < 0.3

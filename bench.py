@@ -167,22 +167,26 @@ def time_device(solver, steps, warmup):
 
 
 def copy_ceiling():
-    """Device-to-device copy of 2 GiB (torch copy_): the practical HBM rate of
-    a pure streaming kernel on this GPU, read + write bytes over its time."""
+    """The practical HBM rate of a pure streaming kernel on this GPU: the
+    engine's device copy (lb_stream_copy, one 8-byte element per thread) of
+    2 GiB, read + write bytes over its time (HIP events, 10 copies)."""
     import torch
+    from jyulb import _native
     a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     a.fill_(1.0)
-    b.copy_(a)
+    st = torch.cuda.current_stream().cuda_stream
+    _native.stream_copy(b.data_ptr(), a.data_ptr(), 2 ** 31, st)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()
     n = 10
     for _ in range(n):
-        b.copy_(a)
+        _native.stream_copy(b.data_ptr(), a.data_ptr(), 2 ** 31, st)
     ev[1].record()
     torch.cuda.synchronize()
     t = ev[0].elapsed_time(ev[1]) / 1e3 / n
+    assert torch.equal(a, b)
     del a, b
     return round(2 * 2 ** 31 / t / 1e9, 1)
 

@@ -265,6 +265,13 @@ int lb_info(lb_solver *s, uint64_t *steps, int *next_odd, uint64_t *device_bytes
  * lb_finalize resolves it). */
 int lb_layout(lb_solver *s, int *layout);
 
+/* Device-to-device copy of `bytes` (no counterpart in the reference): one
+ * 8-byte element per thread, faster than hipMemcpy's device copy on MI355X
+ * (6.4 against 5.0 TB/s on 2 GiB). The engine uses it for the adjoint's
+ * checkpoints and the snapshot slot; bench.py times it as the HBM streaming
+ * reference. Asynchronous on hip_stream (NULL: the default stream). */
+int lb_stream_copy(void *dst, const void *src, size_t bytes, void *hip_stream);
+
 /* Synchronise the handle's stream. */
 int lb_synchronize(lb_solver *s);
 

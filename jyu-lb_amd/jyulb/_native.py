@@ -108,6 +108,7 @@ def lib():
         "lb_evolve_host": ([H, ctypes.c_uint32], ctypes.c_int),
         "lb_store_host_fields": ([H, ctypes.c_int], ctypes.c_int),
         "lb_layout": ([H, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "lb_stream_copy": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p], ctypes.c_int),
         "lb_flow_info_fields_noalloc": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.POINTER(ctypes.c_int32), ctypes.c_double, ctypes.c_void_p,
                                          ctypes.c_void_p], ctypes.c_int),
@@ -132,6 +133,12 @@ def check(rc):
 
 def ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def stream_copy(dst_ptr, src_ptr, nbytes, stream=0):
+    """lb_stream_copy: device-to-device copy on `stream` (device pointers)."""
+    check(lib().lb_stream_copy(ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr), ctypes.c_size_t(int(nbytes)),
+                               ctypes.c_void_p(stream)))
 
 
 class _Pinned:

@@ -138,20 +138,13 @@ LBD void decompose(uint32_t c, int ny, int nzp, int& i, int& j, int& k) {
 // ---------------------------------------------------------------------------
 // step kernel
 // ---------------------------------------------------------------------------
-// code type of the sparse layout's compact odd-step table (see step_sparse):
-// 8 bits measured 4-6% faster odd steps than 16 on the 768x384x384 porous
-// medium; locations beyond an 8-bit offset fall back to the 32-bit table
-#ifndef LB_SP_BITS
-#define LB_SP_BITS 8
-#endif
-#if LB_SP_BITS == 8
-typedef uint8_t SpCode;
-#else
-typedef uint16_t SpCode;
-#endif
-
-// sparse-layout codes (see step_sparse): offset limit, own slot, 32-bit table
-constexpr uint32_t SP_FAR = (1u << (8 * sizeof(SpCode))) - 1u, SP_BOUNCE = SP_FAR - 1u;
+// Sparse-layout location codes of the odd step (see step_sparse): 8 bits per
+// link (measured 4-6% faster odd steps than 16 bits on the 768x384x384 porous
+// medium), four links per 32-bit word, SP_WORDS words per site. A code below
+// SP_BOUNCE is an offset from the wave's base location for the link, SP_BOUNCE
+// names the site's own slot l, SP_FAR a location read from the 32-bit table.
+constexpr int SP_WORDS = 5;
+constexpr uint32_t SP_FAR = 255u, SP_BOUNCE = 254u;
 
 // lattices of at least this many cells deal the step tiles to the XCDs in
 // contiguous ranges (tile_index)
@@ -197,7 +190,7 @@ struct StepParams {
     const uint32_t* cidx;
     const uint32_t* site;
     const uint32_t* tab;
-    const SpCode* tabc;     // compact form of tab (see step_sparse)
+    const uint32_t* tabc;   // compact form of tab (see step_sparse): SP_WORDS rows of nf words
     const uint32_t* tabw;   // its base locations, 18 per wave of 64 compact sites
     uint32_t nf;  // fluid sites = compact sites the step processes
     // dissipation objective (k_diss): per site 19 population seeds, 3

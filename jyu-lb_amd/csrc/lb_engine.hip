@@ -652,7 +652,7 @@ struct lb_solver {
     uint32_t* cidx = nullptr;
     uint32_t* site = nullptr;
     uint32_t* tab = nullptr;
-    SpCode* tabc = nullptr;
+    uint32_t* tabc = nullptr;  // (Q - 1) bytes per compact site (k_sparse_pack)
     uint32_t* tabw = nullptr;
     uint32_t nfp = 0;
     uint8_t* phase = nullptr;
@@ -989,10 +989,11 @@ __global__ void k_sparse_tab(StepParams q, const uint32_t* cidx, uint32_t nfp, u
 // Compact form of the odd-step table, one wave per 64 compact sites: per link
 // the smallest location that is not the site's own slot is the wave's base,
 // and every site stores its offset from it (see step_sparse).
-__global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_t nfp, uint32_t nf, SpCode* tabc,
+__global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_t nfp, uint32_t nf, uint32_t* tabc,
                                                     uint32_t* tabw) {
     const uint32_t w = blockIdx.x;
     const uint32_t n = w * 64 + threadIdx.x;  // < nfp: the grid is nfp / 64 waves
+    uint32_t word[SP_WORDS] = {};
     for (int l = 0; l < Q; ++l) {
         if (l == REST) continue;
         const uint32_t sl = link_slot(l);
@@ -1004,9 +1005,11 @@ __global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_
         uint32_t code = SP_FAR;
         if (bounce) code = SP_BOUNCE;
         else if (live && loc - b < SP_BOUNCE) code = loc - b;
-        tabc[(size_t)sl * nfp + n] = (SpCode)code;
+        word[sl >> 2] |= code << (8 * (sl & 3));
         if (threadIdx.x == 0) tabw[(size_t)w * (Q - 1) + sl] = b == ~0u ? 0u : b;
     }
+    for (int q = 0; q < SP_WORDS - 1; ++q) tabc[(size_t)q * nfp + n] = word[q];
+    ((uint16_t*)(tabc + (size_t)(SP_WORDS - 1) * nfp))[n] = (uint16_t)word[SP_WORDS - 1];
 }
 
 static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
@@ -1382,12 +1385,12 @@ static int build_sparse(lb_solver* s) {
         dfree(s, &s->snap, Q * o * ts);
         dfree(s, (void**)&s->site, o * 4);
         dfree(s, (void**)&s->tab, (Q - 1) * o * 4);
-        dfree(s, (void**)&s->tabc, (Q - 1) * o * sizeof(SpCode));
+        dfree(s, (void**)&s->tabc, (Q - 1) * o);
         dfree(s, (void**)&s->tabw, (Q - 1) * (o / 64) * 4);
         s->nfp = 0;
         if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)) ||
-            (rc = dalloc(s, (void**)&s->tabc, (Q - 1) * nfp * sizeof(SpCode))) ||
+            (rc = dalloc(s, (void**)&s->tabc, (Q - 1) * nfp)) ||
             (rc = dalloc(s, (void**)&s->tabw, (Q - 1) * (nfp / 64) * 4)))
             return rc;
         s->nfp = (uint32_t)nfp;

@@ -340,10 +340,13 @@ __global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
 // neither run the FULL = false kernel, which loads no mask.
 //
 // The step reads the table in a compact form (k_sparse_pack): per wave of 64
-// compact sites and link, a 32-bit base location, and per site an offset
-// from it, SP_BOUNCE for the own slot l, or SP_FAR for a location out of the
-// offset's reach (boundary rows whose upstream row wraps, mirrored links),
-// which is then read from the 32-bit table.
+// compact sites and link, a 32-bit base location, and per site an 8-bit
+// offset from it, SP_BOUNCE for the own slot l, or SP_FAR for a location out
+// of the offset's reach (boundary rows whose upstream row wraps, mirrored
+// links), which is then read from the 32-bit table. A site's 18 codes are
+// four 32-bit words (links slots 0-15, a row of ps words each) and one 16-bit
+// word (slots 16-17): five loads of whole cache lines per wave instead of 18
+// byte loads.
 
 // Odd-step locations of the 18 links of compact site n (loc[REST]: own slot).
 // Codes: an offset from the wave's base for the link (scalar loads), own slot
@@ -353,15 +356,15 @@ __global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
 LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
     const uint32_t ps = (uint32_t)p.ps;
     const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * (Q - 1);
-    uint32_t base[Q - 1], code[Q];
+    uint32_t base[Q - 1], w[SP_WORDS];
 #pragma unroll
     for (int sl = 0; sl < Q - 1; ++sl) base[sl] = B[sl];
     // used here: the loads are not sunk into a branch per link
 #pragma unroll
     for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]));
 #pragma unroll
-    for (int l = 0; l < Q; ++l)
-        if (l != REST) code[l] = p.tabc[link_slot(l) * ps + n];
+    for (int q = 0; q < SP_WORDS - 1; ++q) w[q] = p.tabc[(size_t)q * ps + n];
+    w[SP_WORDS - 1] = ((const uint16_t*)(p.tabc + (size_t)(SP_WORDS - 1) * ps))[n];
     uint32_t far = 0;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -369,8 +372,9 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
             loc[l] = REST * ps + n;
             continue;
         }
-        loc[l] = code[l] == SP_BOUNCE ? (uint32_t)l * ps + n : base[link_slot(l)] + code[l];
-        far |= (code[l] == SP_FAR ? 1u : 0u) << link_slot(l);
+        const uint32_t code = (w[link_slot(l) >> 2] >> (8 * (link_slot(l) & 3))) & 255u;
+        loc[l] = code == SP_BOUNCE ? (uint32_t)l * ps + n : base[link_slot(l)] + code;
+        far |= (code == SP_FAR ? 1u : 0u) << link_slot(l);
     }
     if (__ballot(far != 0)) {
 #pragma unroll
@@ -379,8 +383,9 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
     }
 }
 
+// Step of compact site n; loc: the odd step's locations (sparse_locs)
 template <int P, int RLX, bool ODD, bool FULL, bool DISS = false>
-LBD void step_sparse(const StepParams& p, const uint32_t n) {
+LBD void step_sparse_at(const StepParams& p, const uint32_t n, const uint32_t* loc) {
     using TS = typename Var<P>::TS;
     using TC = typename Var<P>::TC;
     constexpr bool SH = Var<P>::SH;
@@ -397,12 +402,10 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
     // offset n (n * sizeof(TS) < 2^32)
     auto own = [&](int l) { return uniform_ptr(F + (int64_t)l * ps) + n; };
     TC f[Q];
-    uint32_t loc[Q];
     if constexpr (!ODD) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLE>(own(l), p.inv_scale);
     } else {
-        sparse_locs(p, n, loc);
 #pragma unroll
         for (int l = 0; l < Q; ++l) f[l] = ldf<P, NTLO>(l == REST ? own(l) : F + loc[l], p.inv_scale);
     }
@@ -447,6 +450,13 @@ LBD void step_sparse(const StepParams& p, const uint32_t n) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSO>(l == REST ? own(l) : F + loc[l], f[opp(l)], p.scale);
     }
+}
+
+template <int P, int RLX, bool ODD, bool FULL, bool DISS = false>
+LBD void step_sparse(const StepParams& p, const uint32_t n) {
+    uint32_t loc[Q];
+    if constexpr (ODD) sparse_locs(p, n, loc);
+    step_sparse_at<P, RLX, ODD, FULL, DISS>(p, n, loc);
 }
 
 template <int P, int RLX, bool ODD, bool FULL>

@@ -83,6 +83,23 @@ def test_sparse_porous_medium_equals_rows(precision):
         np.testing.assert_array_equal(y, x)
 
 
+@pytest.mark.parametrize("xcd_order", ["0", "18446744073709551615"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp16", "fp32_storage"])
+def test_sparse_large_lattice_both_tile_orders(precision, xcd_order, monkeypatch):
+    """A lattice of 3000+ tiles of 256 compact sites whose last wave holds 3
+    sites (fewer than the 18 links), in the XCD tile order and in the plain
+    one, against the rows layout, bit for bit."""
+    from jyulb.isothermal import Solver
+    monkeypatch.setenv("LB_XCD_REMAP_CELLS", xcd_order)
+    lat, steps = (144, 80, 96), (5, 8)
+    print(f"large sparse {lat} {precision} xcd_cells={xcd_order}", flush=True)
+    a, ph = _porous_run(Solver, lat, steps, precision=precision, layout="rows")
+    b, _ = _porous_run(Solver, lat, steps, precision=precision, layout="sparse")
+    assert a[0] == b[0] == int((ph != 0).sum()) > 3000 * 256 and a[0] % 64 == 3
+    for x, y in zip(a[1:], b[1:]):
+        np.testing.assert_array_equal(y, x)
+
+
 def test_sparse_porous_medium_matches_oracle():
     import jyulb_oracle as jo
     from jyulb.isothermal import Solver

@@ -261,6 +261,13 @@ int lb_restore(lb_solver *s);
  * bytes of device memory held, storage scale of the FP16 variant. */
 int lb_info(lb_solver *s, uint64_t *steps, int *next_odd, uint64_t *device_bytes, double *fp16_scale);
 
+/* Introspection of the sparse layout's odd-step table (no counterpart in the
+ * reference, which has no sparse layout): the number of links, over all
+ * fluid sites, coded as a near offset, the wave's alternate location,
+ * bounce-back, and a full 32-bit location, in counts[0..3]. LB_ESTATE unless
+ * the solver is finalized in the sparse layout. */
+int lb_sparse_codes(lb_solver *s, uint64_t counts[4]);
+
 /* The population layout in use (LB_LAYOUT_*; LB_LAYOUT_AUTO until the first
  * lb_finalize resolves it). */
 int lb_layout(lb_solver *s, int *layout);

@@ -141,10 +141,13 @@ LBD void decompose(uint32_t c, int ny, int nzp, int& i, int& j, int& k) {
 // Sparse-layout location codes of the odd step (see step_sparse): 8 bits per
 // link (measured 4-6% faster odd steps than 16 bits on the 768x384x384 porous
 // medium), four links per 32-bit word, SP_WORDS words per site. A code below
-// SP_BOUNCE is an offset from the wave's base location for the link, SP_BOUNCE
-// names the site's own slot l, SP_FAR a location read from the 32-bit table.
+// SP_ALT is the offset of the location from the wave's base for the link plus
+// the site's lane, SP_ALT names the wave's alternate location for the link,
+// SP_BOUNCE the site's own slot l, SP_FAR a location read from the 32-bit
+// table. Per wave: SP_WAVE_WORDS words, the 18 bases, then the 18 alternates.
 constexpr int SP_WORDS = 5;
-constexpr uint32_t SP_FAR = 255u, SP_BOUNCE = 254u;
+constexpr int SP_WAVE_WORDS = 36;
+constexpr uint32_t SP_FAR = 255u, SP_BOUNCE = 254u, SP_ALT = 253u;
 
 // lattices of at least this many cells deal the step tiles to the XCDs in
 // contiguous ranges (tile_index)
@@ -191,7 +194,7 @@ struct StepParams {
     const uint32_t* site;
     const uint32_t* tab;
     const uint32_t* tabc;   // compact form of tab (see step_sparse): SP_WORDS rows of nf words
-    const uint32_t* tabw;   // its base locations, 18 per wave of 64 compact sites
+    const uint32_t* tabw;   // its base and alternate locations, SP_WAVE_WORDS per wave of 64 compact sites
     uint32_t nf;  // fluid sites = compact sites the step processes
     // dissipation objective (k_diss): per site 19 population seeds, 3
     // acceleration seeds and the site's Pi^neq:Pi^neq, each an nq-long row of

@@ -654,6 +654,7 @@ struct lb_solver {
     uint32_t* tab = nullptr;
     uint32_t* tabc = nullptr;  // (Q - 1) bytes per compact site (k_sparse_pack)
     uint32_t* tabw = nullptr;
+    uint64_t sp_kinds[4] = {0, 0, 0, 0};  // link codes by kind (lb_sparse_codes)
     uint32_t nfp = 0;
     uint8_t* phase = nullptr;
     double* den = nullptr;
@@ -986,27 +987,56 @@ __global__ void k_sparse_tab(StepParams q, const uint32_t* cidx, uint32_t nfp, u
     }
 }
 
-// Compact form of the odd-step table, one wave per 64 compact sites: per link
-// the smallest location that is not the site's own slot is the wave's base,
-// and every site stores its offset from it (see step_sparse).
+// Compact form of the odd-step table, one wave per 64 compact sites (see
+// sparse_locs). Per link, over the wave's lanes that do not bounce back, the
+// base is the value of loc - lane (modulo 2^32) whose window [base, base +
+// SP_ALT) holds the most of them (the lowest such lane on ties); the
+// alternate is the location of the lowest lane outside it, and every lane
+// with that location takes SP_ALT.
+// Counts of the link codes go to kinds[0..3]: near, SP_ALT, SP_BOUNCE, SP_FAR.
 __global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_t nfp, uint32_t nf, uint32_t* tabc,
-                                                    uint32_t* tabw) {
-    const uint32_t w = blockIdx.x;
-    const uint32_t n = w * 64 + threadIdx.x;  // < nfp: the grid is nfp / 64 waves
+                                                    uint32_t* tabw, unsigned long long* kinds) {
+    const uint32_t w = blockIdx.x, lane = threadIdx.x;
+    const uint32_t n = w * 64 + lane;  // < nfp: the grid is nfp / 64 waves
+    const bool live = n < nf;
     uint32_t word[SP_WORDS] = {};
     for (int l = 0; l < Q; ++l) {
         if (l == REST) continue;
         const uint32_t sl = link_slot(l);
         const uint32_t loc = tab[(size_t)sl * nfp + n];
-        const bool live = n < nf;
         const bool bounce = live && loc == (uint32_t)l * nfp + n;
-        uint32_t b = (live && !bounce) ? loc : ~0u;
-        for (int o = 32; o > 0; o >>= 1) b = min(b, (uint32_t)__shfl_xor((int)b, o, 64));
-        uint32_t code = SP_FAR;
+        const bool cand = live && !bounce;
+        const uint32_t d = loc - lane;
+        const uint64_t cands = __ballot(cand);
+        int best = -1;
+        uint32_t base = 0;
+        for (int c = 0; c < 64; ++c) {
+            const uint32_t dc = (uint32_t)__shfl((int)d, c, 64);
+            if (!((cands >> c) & 1u)) continue;
+            const int cnt = __popcll(__ballot(cand && d - dc < SP_ALT));
+            if (cnt > best) {
+                best = cnt;
+                base = dc;
+            }
+        }
+        const bool in = cand && d - base < SP_ALT;
+        const uint64_t out = __ballot(cand && !in);
+        const uint32_t alt = out ? (uint32_t)__shfl((int)loc, __ffsll((unsigned long long)out) - 1, 64) : 0u;
+        uint32_t code = SP_FAR;  // also for lanes past the last site, which no step reads
         if (bounce) code = SP_BOUNCE;
-        else if (live && loc - b < SP_BOUNCE) code = loc - b;
+        else if (in) code = d - base;
+        else if (cand && loc == alt) code = SP_ALT;
         word[sl >> 2] |= code << (8 * (sl & 3));
-        if (threadIdx.x == 0) tabw[(size_t)w * (Q - 1) + sl] = b == ~0u ? 0u : b;
+        const unsigned kn[4] = {(unsigned)__popcll(__ballot(in)),
+                                (unsigned)__popcll(__ballot(cand && !in && code == SP_ALT)),
+                                (unsigned)__popcll(__ballot(bounce)),
+                                (unsigned)__popcll(__ballot(cand && code == SP_FAR))};
+        if (lane == 0) {
+            tabw[(size_t)w * SP_WAVE_WORDS + sl] = base;
+            tabw[(size_t)w * SP_WAVE_WORDS + Q - 1 + sl] = alt;
+            for (int k = 0; k < 4; ++k)
+                if (kn[k]) atomicAdd(kinds + k, (unsigned long long)kn[k]);
+        }
     }
     for (int q = 0; q < SP_WORDS - 1; ++q) tabc[(size_t)q * nfp + n] = word[q];
     ((uint16_t*)(tabc + (size_t)(SP_WORDS - 1) * nfp))[n] = (uint16_t)word[SP_WORDS - 1];
@@ -1386,12 +1416,12 @@ static int build_sparse(lb_solver* s) {
         dfree(s, (void**)&s->site, o * 4);
         dfree(s, (void**)&s->tab, (Q - 1) * o * 4);
         dfree(s, (void**)&s->tabc, (Q - 1) * o);
-        dfree(s, (void**)&s->tabw, (Q - 1) * (o / 64) * 4);
+        dfree(s, (void**)&s->tabw, SP_WAVE_WORDS * (o / 64) * 4);
         s->nfp = 0;
         if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tabc, (Q - 1) * nfp)) ||
-            (rc = dalloc(s, (void**)&s->tabw, (Q - 1) * (nfp / 64) * 4)))
+            (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)))
             return rc;
         s->nfp = (uint32_t)nfp;
     }
@@ -1439,9 +1469,12 @@ static int build_sparse(lb_solver* s) {
     HIPCHK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (herr) return fail(LB_EHIP, "sparse layout set-up: a link resolves to a non-fluid site");
+    unsigned long long* kinds = (unsigned long long*)s->red;
+    HIPCHK(hipMemsetAsync(kinds, 0, sizeof(s->sp_kinds), s->stream));
     k_sparse_pack<<<(unsigned)(nfp / 64), 64, 0, s->stream>>>(s->tab, (uint32_t)nfp, (uint32_t)nf, s->tabc,
-                                                             s->tabw);
+                                                             s->tabw, kinds);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s->sp_kinds, kinds, sizeof(s->sp_kinds), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return LB_OK;
 }
@@ -2308,6 +2341,14 @@ int lb_info(lb_solver* s, uint64_t* steps, int* next_odd, uint64_t* bytes, doubl
     if (next_odd) *next_odd = s->next_odd;
     if (bytes) *bytes = s->bytes;
     if (scale) *scale = s->scale;
+    return LB_OK;
+}
+
+int lb_sparse_codes(lb_solver* s, uint64_t* counts) {
+    if (!s || !counts) return fail(LB_EINVAL, "null argument");
+    if (!s->finalized || s->layout != LB_LAYOUT_SPARSE)
+        return fail(LB_ESTATE, "lb_sparse_codes needs a finalized solver in the sparse layout");
+    for (int k = 0; k < 4; ++k) counts[k] = s->sp_kinds[k];
     return LB_OK;
 }
 

@@ -339,32 +339,38 @@ __global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
 // moving-wall or Zou-He bits, read from the dense mask; lattices with
 // neither run the FULL = false kernel, which loads no mask.
 //
-// The step reads the table in a compact form (k_sparse_pack): per wave of 64
-// compact sites and link, a 32-bit base location, and per site an 8-bit
-// offset from it, SP_BOUNCE for the own slot l, or SP_FAR for a location out
-// of the offset's reach (boundary rows whose upstream row wraps, mirrored
-// links), which is then read from the 32-bit table. A site's 18 codes are
-// four 32-bit words (links slots 0-15, a row of ps words each) and one 16-bit
-// word (slots 16-17): five loads of whole cache lines per wave instead of 18
-// byte loads.
+// The step reads the table in a compact form (k_sparse_pack). Per wave of 64
+// compact sites and link: a base and an alternate location (32 bits each).
+// Per site and link an 8-bit code: below SP_ALT, the location is base + lane
+// + code (in a run of fluid sites the upstream locations advance with the
+// lane, so the codes stay small); SP_ALT, the wave's alternate location (it
+// takes the one outlier of most waves: the lane whose link wraps around a
+// periodic face at the end of a row); SP_BOUNCE, the own slot l; SP_FAR, a
+// location read from the 32-bit table (links of under 1% of the waves on a
+// random-sphere medium). A site's 18 codes are four 32-bit words (link slots
+// 0-15, a row of ps words each) and one 16-bit word (slots 16-17): five
+// loads of whole cache lines per wave.
 
 // Odd-step locations of the 18 links of compact site n (loc[REST]: own slot).
-// Codes: an offset from the wave's base for the link (scalar loads), own slot
-// l (bounce-back), or the full 32-bit location. They are decoded with
-// selects, so that all 18 population loads after this can be in flight
-// together; the rare SP_FAR links are fixed up in one wave-uniform branch.
+// The codes are decoded with selects, so that all 18 population loads after
+// this can be in flight together; the rare SP_FAR links are fixed up in one
+// wave-uniform branch.
 LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
     const uint32_t ps = (uint32_t)p.ps;
-    const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * (Q - 1);
-    uint32_t base[Q - 1], w[SP_WORDS];
+    const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * SP_WAVE_WORDS;
+    uint32_t base[Q - 1], alt[Q - 1], w[SP_WORDS];
 #pragma unroll
-    for (int sl = 0; sl < Q - 1; ++sl) base[sl] = B[sl];
+    for (int sl = 0; sl < Q - 1; ++sl) {
+        base[sl] = B[sl];
+        alt[sl] = B[Q - 1 + sl];
+    }
     // used here: the loads are not sunk into a branch per link
 #pragma unroll
-    for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]));
+    for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]), "s"(alt[sl]));
 #pragma unroll
     for (int q = 0; q < SP_WORDS - 1; ++q) w[q] = p.tabc[(size_t)q * ps + n];
     w[SP_WORDS - 1] = ((const uint16_t*)(p.tabc + (size_t)(SP_WORDS - 1) * ps))[n];
+    const uint32_t lane = n & 63u;  // waves start at multiples of 64
     uint32_t far = 0;
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
@@ -372,9 +378,11 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
             loc[l] = REST * ps + n;
             continue;
         }
-        const uint32_t code = (w[link_slot(l) >> 2] >> (8 * (link_slot(l) & 3))) & 255u;
-        loc[l] = code == SP_BOUNCE ? (uint32_t)l * ps + n : base[link_slot(l)] + code;
-        far |= (code == SP_FAR ? 1u : 0u) << link_slot(l);
+        const int sl = link_slot(l);
+        const uint32_t code = (w[sl >> 2] >> (8 * (sl & 3))) & 255u;
+        const uint32_t near = base[sl] + lane + code;  // modulo 2^32: the base may be below 0
+        loc[l] = code < SP_ALT ? near : code == SP_ALT ? alt[sl] : (uint32_t)l * ps + n;
+        far |= (code == SP_FAR ? 1u : 0u) << sl;
     }
     if (__ballot(far != 0)) {
 #pragma unroll

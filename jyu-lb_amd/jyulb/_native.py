@@ -91,6 +91,7 @@ def lib():
         "lb_download_den_vel_device": ([H, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "lb_info": ([H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int),
                      ctypes.POINTER(ctypes.c_uint64), dp], ctypes.c_int),
+        "lb_sparse_codes": ([H, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
         "lb_synchronize": ([H], ctypes.c_int),
         "lb_set_face_data_device": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "lb_objective": ([H, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
@@ -314,6 +315,13 @@ class Engine:
         st, odd, by, sc = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint64(), ctypes.c_double()
         self.call("lb_info", ctypes.byref(st), ctypes.byref(odd), ctypes.byref(by), ctypes.byref(sc))
         return dict(steps=st.value, next_odd=odd.value, device_bytes=by.value, fp16_scale=sc.value)
+
+    def sparse_codes(self):
+        """Odd-step links of the sparse layout by code kind: near offsets, the
+        wave's alternate location, bounce-back, full 32-bit locations."""
+        c = (ctypes.c_uint64 * 4)()
+        self.call("lb_sparse_codes", c)
+        return dict(zip(("near", "alt", "bounce", "far"), (int(x) for x in c)))
 
     def layout(self):
         """Name of the population layout in use ("auto" until finalize resolves it)."""

@@ -100,6 +100,27 @@ def test_sparse_large_lattice_both_tile_orders(precision, xcd_order, monkeypatch
         np.testing.assert_array_equal(y, x)
 
 
+def test_sparse_code_kinds():
+    """The odd-step table of a porous medium with periodic x/z and symmetric
+    y faces holds every kind of code: near offsets for most links, the wave's
+    alternate location (rows that wrap around the periodic z face), bounce-
+    back at the spheres, and full 32-bit locations (mirrored links at the
+    symmetric faces); the kinds add up to 18 links per fluid site."""
+    from jyulb.isothermal import Solver
+    from jyulb.defs import TRT, PERIODIC, SYMMETRIC
+    from jyulb.geometry import porous_spheres
+    lat = np.array([48, 40, 90], dtype=np.int32)
+    bc = np.array([PERIODIC, PERIODIC, SYMMETRIC, SYMMETRIC, PERIODIC, PERIODIC], dtype=np.int32)
+    s = Solver(lat, TRT, 1.0 / 30.0, np.zeros(3), bc, layout="sparse")
+    s.get_phase()[...] = np.where(porous_spheres(lat, 3, 6, 0.25, seed=11), 0, 1)
+    nf = s.finalize_init()
+    k = s.engine.sparse_codes()
+    print("code kinds", k, "fluid sites", nf, flush=True)
+    assert sum(k.values()) == 18 * nf
+    assert min(k.values()) > 0
+    assert k["near"] > 0.9 * 18 * nf and k["far"] < 0.02 * 18 * nf
+
+
 def test_sparse_porous_medium_matches_oracle():
     import jyulb_oracle as jo
     from jyulb.isothermal import Solver
@@ -164,8 +185,8 @@ def test_sparse_device_memory_follows_fluid_sites():
     ncell = 64 * 32 * 64
     # rows: 19 f64 per cell; sparse: per padded fluid site 19 f64, its site
     # index, the 18 u32 locations and their 8-bit codes, per wave of 64 the 18
-    # u32 bases, and a compact index per cell
-    sparse = (19 * 8 + 4 + 18 * 4 + 18) * nfp + 18 * 4 * (nfp // 64) + 4 * ncell
+    # u32 bases and 18 alternates, and a compact index per cell
+    sparse = (19 * 8 + 4 + 18 * 4 + 18) * nfp + 36 * 4 * (nfp // 64) + 4 * ncell
     assert b_rows - b_sp == 19 * 8 * ncell - sparse
 
 

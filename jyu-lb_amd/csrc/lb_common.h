@@ -138,16 +138,20 @@ LBD void decompose(uint32_t c, int ny, int nzp, int& i, int& j, int& k) {
 // ---------------------------------------------------------------------------
 // step kernel
 // ---------------------------------------------------------------------------
-// Sparse-layout location codes of the odd step (see step_sparse): 8 bits per
-// link (measured 4-6% faster odd steps than 16 bits on the 768x384x384 porous
-// medium), four links per 32-bit word, SP_WORDS words per site. A code below
-// SP_ALT is the offset of the location from the wave's base for the link plus
-// the site's lane, SP_ALT names the wave's alternate location for the link,
-// SP_BOUNCE the site's own slot l, SP_FAR a location read from the 32-bit
-// table. Per wave: SP_WAVE_WORDS words, the 18 bases, then the 18 alternates.
-constexpr int SP_WORDS = 5;
+// Sparse-layout location codes of the odd step (see step_sparse): SP_BITS
+// bits per link, SP_PER_WORD links per 32-bit word, SP_WORDS words per site. A
+// code below SP_ALT is the offset of the location from the wave's base for
+// the link plus the site's lane, SP_ALT names the wave's alternate location
+// for the link, SP_BOUNCE the site's own slot l, SP_FAR a location read from
+// the 32-bit table. Per wave: SP_WAVE_WORDS words, the 18 bases, then the 18
+// alternates.
+constexpr int SP_BITS = 5, SP_PER_WORD = 6, SP_WORDS = 3;
 constexpr int SP_WAVE_WORDS = 36;
-constexpr uint32_t SP_FAR = 255u, SP_BOUNCE = 254u, SP_ALT = 253u;
+constexpr uint32_t SP_FAR = 31u, SP_BOUNCE = 30u, SP_ALT = 29u;
+static_assert(SP_PER_WORD * SP_BITS <= 32 && SP_PER_WORD * SP_WORDS == 18, "code packing");
+LBD uint32_t sp_code(const uint32_t* w, int sl) {
+    return (w[sl / SP_PER_WORD] >> (SP_BITS * (sl % SP_PER_WORD))) & ((1u << SP_BITS) - 1u);
+}
 
 // lattices of at least this many cells deal the step tiles to the XCDs in
 // contiguous ranges (tile_index)

@@ -652,7 +652,7 @@ struct lb_solver {
     uint32_t* cidx = nullptr;
     uint32_t* site = nullptr;
     uint32_t* tab = nullptr;
-    uint32_t* tabc = nullptr;  // (Q - 1) bytes per compact site (k_sparse_pack)
+    uint32_t* tabc = nullptr;  // SP_WORDS words per compact site (k_sparse_pack)
     uint32_t* tabw = nullptr;
     uint64_t sp_kinds[4] = {0, 0, 0, 0};  // link codes by kind (lb_sparse_codes)
     uint32_t nfp = 0;
@@ -1026,7 +1026,7 @@ __global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_
         if (bounce) code = SP_BOUNCE;
         else if (in) code = d - base;
         else if (cand && loc == alt) code = SP_ALT;
-        word[sl >> 2] |= code << (8 * (sl & 3));
+        word[sl / SP_PER_WORD] |= code << (SP_BITS * (sl % SP_PER_WORD));
         const unsigned kn[4] = {(unsigned)__popcll(__ballot(in)),
                                 (unsigned)__popcll(__ballot(cand && !in && code == SP_ALT)),
                                 (unsigned)__popcll(__ballot(bounce)),
@@ -1038,8 +1038,7 @@ __global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_
                 if (kn[k]) atomicAdd(kinds + k, (unsigned long long)kn[k]);
         }
     }
-    for (int q = 0; q < SP_WORDS - 1; ++q) tabc[(size_t)q * nfp + n] = word[q];
-    ((uint16_t*)(tabc + (size_t)(SP_WORDS - 1) * nfp))[n] = (uint16_t)word[SP_WORDS - 1];
+    for (int q = 0; q < SP_WORDS; ++q) tabc[(size_t)q * nfp + n] = word[q];
 }
 
 static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
@@ -1415,12 +1414,12 @@ static int build_sparse(lb_solver* s) {
         dfree(s, &s->snap, Q * o * ts);
         dfree(s, (void**)&s->site, o * 4);
         dfree(s, (void**)&s->tab, (Q - 1) * o * 4);
-        dfree(s, (void**)&s->tabc, (Q - 1) * o);
+        dfree(s, (void**)&s->tabc, SP_WORDS * o * 4);
         dfree(s, (void**)&s->tabw, SP_WAVE_WORDS * (o / 64) * 4);
         s->nfp = 0;
         if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)) ||
-            (rc = dalloc(s, (void**)&s->tabc, (Q - 1) * nfp)) ||
+            (rc = dalloc(s, (void**)&s->tabc, SP_WORDS * nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)))
             return rc;
         s->nfp = (uint32_t)nfp;

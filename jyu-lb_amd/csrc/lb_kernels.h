@@ -341,15 +341,15 @@ __global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
 //
 // The step reads the table in a compact form (k_sparse_pack). Per wave of 64
 // compact sites and link: a base and an alternate location (32 bits each).
-// Per site and link an 8-bit code: below SP_ALT, the location is base + lane
+// Per site and link a 5-bit code: below SP_ALT, the location is base + lane
 // + code (in a run of fluid sites the upstream locations advance with the
 // lane, so the codes stay small); SP_ALT, the wave's alternate location (it
 // takes the one outlier of most waves: the lane whose link wraps around a
 // periodic face at the end of a row); SP_BOUNCE, the own slot l; SP_FAR, a
-// location read from the 32-bit table (links of under 1% of the waves on a
-// random-sphere medium). A site's 18 codes are four 32-bit words (link slots
-// 0-15, a row of ps words each) and one 16-bit word (slots 16-17): five
-// loads of whole cache lines per wave.
+// location read from the 32-bit table (links of about 1% of the waves on a
+// random-sphere medium). A site's 18 codes are three 32-bit words, six links
+// each, in rows of ps words: three loads of whole cache lines per wave, 12
+// bytes per site (8-bit codes, 18 bytes, measured 1-4% slower).
 
 // Odd-step locations of the 18 links of compact site n (loc[REST]: own slot).
 // The codes are decoded with selects, so that all 18 population loads after
@@ -368,8 +368,7 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
 #pragma unroll
     for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]), "s"(alt[sl]));
 #pragma unroll
-    for (int q = 0; q < SP_WORDS - 1; ++q) w[q] = p.tabc[(size_t)q * ps + n];
-    w[SP_WORDS - 1] = ((const uint16_t*)(p.tabc + (size_t)(SP_WORDS - 1) * ps))[n];
+    for (int q = 0; q < SP_WORDS; ++q) w[q] = p.tabc[(size_t)q * ps + n];
     const uint32_t lane = n & 63u;  // waves start at multiples of 64
     uint32_t far = 0;
 #pragma unroll
@@ -379,7 +378,7 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
             continue;
         }
         const int sl = link_slot(l);
-        const uint32_t code = (w[sl >> 2] >> (8 * (sl & 3))) & 255u;
+        const uint32_t code = sp_code(w, sl);
         const uint32_t near = base[sl] + lane + code;  // modulo 2^32: the base may be below 0
         loc[l] = code < SP_ALT ? near : code == SP_ALT ? alt[sl] : (uint32_t)l * ps + n;
         far |= (code == SP_FAR ? 1u : 0u) << sl;

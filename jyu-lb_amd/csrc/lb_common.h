@@ -159,6 +159,8 @@ LBD uint32_t sp_code(const uint32_t* w, int sl) {
 #define LB_XCD_REMAP_CELLS (1u << 25)
 #endif
 
+constexpr uint32_t XCD_CONTIGUOUS = 0xffffffffu;
+
 struct StepParams {
     void* f;
     const uint32_t* mask;
@@ -180,7 +182,7 @@ struct StepParams {
     int64_t rs;  // element stride between rows (i,j) of the population array
     int64_t ps;  // element stride between populations
     const uint8_t* wclass;  // per wave of 64 sites: 0 = plain (see k_mask)
-    uint32_t xcd;           // deal tiles to XCDs in contiguous ranges (tile_index)
+    uint32_t xcd;           // tile order (tile_index): 0 round-robin, XCD_CONTIGUOUS, or a chunk of tiles per XCD
     uint32_t lds[2];        // dynamic LDS per workgroup of the bulk step launch (even, odd): caps occupancy
     uint32_t tile;          // threads per workgroup of the launch that reads ntiles (see TILE)
     const uint32_t* mov_idx;  // per site: index of a moving site in ladd_u

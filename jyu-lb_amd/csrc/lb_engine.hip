@@ -631,6 +631,7 @@ struct lb_solver {
     double scale;
     hipStream_t stream = nullptr;
     uint64_t xcd_cells = LB_XCD_REMAP_CELLS;  // tile order threshold (tile_index)
+    uint32_t xcd_chunk = 0;                   // tiles per XCD chunk in that order (0: contiguous ranges)
     uint32_t step_lds[2] = {0, 0};            // see StepParams::lds
     uint32_t step_tile = TILE;                // see StepParams::tile
     bool tune = false;                        // choose the step launch on the next long evolve (tune_step)
@@ -757,7 +758,8 @@ static StepParams params(const lb_solver* s) {
     p.nq = s->nq;
     p.tile = s->step_tile;
     p.ntiles = ((s->sparse ? s->nfp : s->nq) + p.tile - 1) / p.tile;
-    p.xcd = s->nq >= s->xcd_cells ? 1u : 0u;  // the tuned launch may set xcd_cells to 0 or UINT64_MAX
+    // the tuned launch may set xcd_cells to 0 or UINT64_MAX
+    p.xcd = s->nq >= s->xcd_cells ? (s->xcd_chunk ? s->xcd_chunk : XCD_CONTIGUOUS) : 0u;
     p.lds[0] = s->step_lds[0];
     p.lds[1] = s->step_lds[1];
     p.rs = s->rs;
@@ -1152,10 +1154,20 @@ int lb_device_count(void) {
 // Launch defaults of a population layout, then the environment overrides.
 static void layout_defaults(lb_solver* s, int layout) {
     const bool sparse = layout == LB_LAYOUT_SPARSE;
-    // The sparse FP32-storage kernels measured 5% faster on the 768x384x384
-    // medium in the plain round-robin order (the sparse FP64 kernel 4% faster
-    // with the XCD order, FP16 the same either way).
-    if (sparse && (s->prec == LB_FP32_SHIFTED || s->prec == LB_FP32_STORAGE)) s->xcd_cells = UINT64_MAX;
+    // The sparse step on the 768x384x384 medium, by tile order (ms per step,
+    // FP64 / FP32 / FP16 / FP32 storage): round-robin 6.29 / 2.86 / 1.66 /
+    // 2.81, contiguous ranges per XCD 5.90 / 3.03 / 1.66 / 3.05, and chunks
+    // of 32 / 4 / 8 / 2 tiles dealt to the XCDs in turn 5.49 / 2.82 / 1.57 /
+    // 2.79 (chunks of 3, 6, 12 or 24 tiles ran 1-4% slower than the powers
+    // of two beside them). The dense layouts keep contiguous ranges: chunks
+    // were within +-1.5% on the 256^3 cavity and the 512x128x128 duct and
+    // 2-7% slower on the medium.
+    const uint32_t chunk[4] = {32, 4, 8, 2};  // by LB_FP64, LB_FP32_SHIFTED, LB_FP16_STORAGE, LB_FP32_STORAGE
+    s->xcd_chunk = sparse ? chunk[s->prec] : 0u;
+    if (const char* env = getenv("LB_XCD_CHUNK")) {
+        s->xcd_chunk = (uint32_t)strtoul(env, nullptr, 10);
+        s->tune_xcd = false;
+    }
     if (const char* env = getenv("LB_XCD_REMAP_CELLS")) {
         s->xcd_cells = strtoull(env, nullptr, 10);
         s->tune_xcd = false;
@@ -1640,6 +1652,7 @@ static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
 struct StepCfg {
     uint32_t tile, lds_even, lds_odd;
     uint64_t xcd_cells;  // see tile_index: 0 = XCD order, UINT64_MAX = round-robin
+    uint32_t chunk;      // XCD order: tiles per chunk (0: contiguous ranges)
 };
 constexpr int TUNE_WARM = 2, TUNE_STEPS = 4;
 constexpr float TUNE_MARGIN = 0.02f;
@@ -1651,28 +1664,33 @@ constexpr int TUNE_MAX = 10;
 static int tune_candidates(const lb_solver* s, StepCfg* c) {
     int n = 0;
     const uint64_t xd = s->nq >= s->xcd_cells ? 0 : UINT64_MAX;  // the default order
-    c[n++] = {s->step_tile, s->step_lds[0], s->step_lds[1], xd};
+    const uint32_t ch = s->xcd_chunk;
+    c[n++] = {s->step_tile, s->step_lds[0], s->step_lds[1], xd, ch};
     auto add = [&](StepCfg x) {
+        if (x.xcd_cells) x.chunk = ch;  // unused in round-robin order
         for (int i = 0; i < n; ++i)
             if (c[i].tile == x.tile && c[i].lds_even == x.lds_even && c[i].lds_odd == x.lds_odd &&
-                c[i].xcd_cells == x.xcd_cells)
+                c[i].xcd_cells == x.xcd_cells && c[i].chunk == x.chunk)
                 return;
         if (n < TUNE_MAX) c[n++] = x;
     };
     if (!s->sparse) {
         if (s->prec == LB_FP64 || s->prec == LB_FP32_STORAGE) {
-            add({512, 0, 0, xd});
-            add({256, 0, 0, xd});
-            add({256, 40000, 40000, xd});  // 4 workgroups per CU: 4 waves per SIMD
-            add({256, 50000, 50000, xd});  // 3 waves per SIMD
+            add({512, 0, 0, xd, ch});
+            add({256, 0, 0, xd, ch});
+            add({256, 40000, 40000, xd, ch});  // 4 workgroups per CU: 4 waves per SIMD
+            add({256, 50000, 50000, xd, ch});  // 3 waves per SIMD
         } else if (s->prec == LB_FP32_SHIFTED) {
-            add({256, 0, 0, xd});
-            add({256, 26000, 0, xd});
+            add({256, 0, 0, xd, ch});
+            add({256, 26000, 0, xd, ch});
         }
     }
-    if (s->tune_xcd && (uint64_t)s->nq >= TUNE_XCD_CELLS) {  // each of them in the other tile order
+    if (s->tune_xcd && (uint64_t)s->nq >= TUNE_XCD_CELLS) {  // each of them in the other tile orders
         const int m = n;
-        for (int i = 0; i < m; ++i) add({c[i].tile, c[i].lds_even, c[i].lds_odd, xd ? 0 : UINT64_MAX});
+        for (int i = 0; i < m; ++i) {
+            add({c[i].tile, c[i].lds_even, c[i].lds_odd, xd ? 0 : UINT64_MAX, ch});
+            if (ch) add({c[i].tile, c[i].lds_even, c[i].lds_odd, 0, 0});  // contiguous ranges
+        }
     }
     return n;
 }
@@ -1697,6 +1715,7 @@ static int tune_step(lb_solver* s) {
         s->step_lds[0] = c[i].lds_even;
         s->step_lds[1] = c[i].lds_odd;
         s->xcd_cells = c[i].xcd_cells;
+        s->xcd_chunk = c[i].chunk;
         HIPCHK(hipEventRecord(ev[2 * i], s->stream));
         rc = run_steps(s, TUNE_STEPS, false);
         HIPCHK(hipEventRecord(ev[2 * i + 1], s->stream));
@@ -1718,8 +1737,9 @@ static int tune_step(lb_solver* s) {
             tbest = ms;
         }
         if (s->tune_verbose)
-            fprintf(stderr, "[jyulb] step launch tile %u, LDS %u/%u, %s order: %.4f ms per step\n", c[i].tile,
-                    c[i].lds_even, c[i].lds_odd, c[i].xcd_cells ? "round-robin" : "XCD", ms / TUNE_STEPS);
+            fprintf(stderr, "[jyulb] step launch tile %u, LDS %u/%u, %s order (chunk %u): %.4f ms per step\n",
+                    c[i].tile, c[i].lds_even, c[i].lds_odd, c[i].xcd_cells ? "round-robin" : "XCD", c[i].chunk,
+                    ms / TUNE_STEPS);
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     // a few timed steps cannot separate near-ties: the default stays unless
@@ -1731,6 +1751,7 @@ static int tune_step(lb_solver* s) {
     s->step_lds[0] = c[best].lds_even;
     s->step_lds[1] = c[best].lds_odd;
     s->xcd_cells = c[best].xcd_cells;
+    s->xcd_chunk = c[best].chunk;
     return LB_OK;
 }
 

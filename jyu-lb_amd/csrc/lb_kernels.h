@@ -257,10 +257,21 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
 // that name, read at lb_create, overrides it): the 768x384x384 porous
 // medium ran 5-7% faster with it, while the 256^3 cavity, the 512x128x128
 // duct and the 128x64x64 porous channel ran 1.5-3.8% faster without.
+// With p.xcd = C other than XCD_CONTIGUOUS, the XCDs take chunks of C
+// consecutive tiles in turn (chunk m to XCD m % 8) and the grid is padded to
+// a multiple of 8C.
 LBD uint32_t tile_index(const StepParams& p) {
     if (!p.xcd) return blockIdx.x;
-    const uint32_t per = gridDim.x >> 3;
-    return (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    const uint32_t x = blockIdx.x & 7u, i = blockIdx.x >> 3;
+    const uint32_t c = p.xcd == XCD_CONTIGUOUS ? gridDim.x >> 3 : p.xcd;
+    return ((i / c) * 8 + x) * c + i % c;
+}
+
+// grid of a launch that reads tile_index
+inline unsigned tile_grid(const StepParams& p) {
+    if (!p.xcd) return p.ntiles;
+    const unsigned q = p.xcd == XCD_CONTIGUOUS ? 8u : 8u * p.xcd;
+    return (p.ntiles + q - 1) / q * q;
 }
 
 // Mask bits that send a site to the special-site kernel. An even step reads
@@ -784,7 +795,7 @@ __global__ void k_post_f(StepParams p, int next_odd, double* out) {
 // ---------------------------------------------------------------------------
 template <int P, int RLX>
 static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
-    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    const unsigned grid = tile_grid(p);
     if (p.tab) {  // sparse layout: one kernel, with the mask only where needed
         if (p.moving || p.nzh) {
             if (odd) k_step_sp<P, RLX, true, true><<<grid, p.tile, p.lds[1], st>>>(p);
@@ -822,7 +833,7 @@ static void launch_adj(const StepParams& ps, const AdjParams& x, bool odd, hipSt
     StepParams p = ps;  // the adjoint kernels run TILE threads per workgroup
     p.tile = TILE;
     p.ntiles = ((p.tab ? (uint32_t)p.ps : p.nq) + TILE - 1) / TILE;
-    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    const unsigned grid = tile_grid(p);
     if (p.tab) {  // sparse layout
         if (p.nzh) {
             if (odd) k_adj_sp<P, RLX, true, true><<<grid, TILE, 0, st>>>(p, x);
@@ -853,7 +864,7 @@ void launch_adj_rlx(const StepParams& p, const AdjParams& x, int rlx, bool odd, 
 // the dissipation objective's gather before the step of parity `odd`
 template <int P, int RLX>
 static void launch_diss(const StepParams& p, bool odd, hipStream_t st) {
-    const unsigned grid = p.xcd ? (p.ntiles + 7) / 8 * 8 : p.ntiles;
+    const unsigned grid = tile_grid(p);
     if (p.tab) {  // sparse layout
         if (p.moving || p.nzh) {
             if (odd) k_diss_sp<P, RLX, true, true><<<grid, p.tile, 0, st>>>(p);

@@ -211,6 +211,8 @@ def headline(precs, steps, warmup, layout, solid):
                          frac=round(gbs / PEAK_HBM_GBS, 4), layout=lay, kernel=step_kernel(prec, lay),
                          nfluid=nfl, device_bytes=int(s.engine.info()["device_bytes"]),
                          fp16_scale=s.engine.info()["fp16_scale"])
+        if lay == "sparse":
+            res[prec]["link_codes"] = s.engine.sparse_codes()
         print(f"[bench] {prec} {lay}: {res[prec]['mlups']} MLUPS, {res[prec]['ms_per_step']} ms/step, "
               f"{gbs:.0f} GB/s algorithmic", file=sys.stderr, flush=True)
         del s

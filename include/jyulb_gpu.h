@@ -49,7 +49,7 @@ extern "C" {
 #define LB_LAYOUT_SPARSE 3      /* fluid sites only: f[l][n] over the compacted fluid sites, with a table of
                                    the 18 odd-step locations per site (the fluid-only AA arrays of
                                    include/kernel/kernel.cpp:227-271), built on the device and read
-                                   as 8-bit codes from per-wave bases; allocated by lb_finalize */
+                                   as 5-bit codes against per-wave bases; allocated by lb_finalize */
 #define LB_LAYOUT_AUTO 4        /* chosen at the first lb_finalize: SPARSE when at least 5% of the cells are
                                    solid (LB_AUTO_SPARSE_SOLID in the environment sets the fraction), ROWS
                                    otherwise; results are identical in every layout */
@@ -274,7 +274,7 @@ int lb_layout(lb_solver *s, int *layout);
 
 /* Device-to-device copy of `bytes` (no counterpart in the reference): one
  * 8-byte element per thread, faster than hipMemcpy's device copy on MI355X
- * (6.4 against 5.0 TB/s on 2 GiB). The engine uses it for the adjoint's
+ * (5.8 against 4.9 TB/s, read + write, on 2 GiB). The engine uses it for the adjoint's
  * checkpoints and the snapshot slot; bench.py times it as the HBM streaming
  * reference. Asynchronous on hip_stream (NULL: the default stream). */
 int lb_stream_copy(void *dst, const void *src, size_t bytes, void *hip_stream);

@@ -113,8 +113,10 @@ __global__ void k_export_vec3(const TC* src, double* out, Dims d, uint32_t nq) {
 }
 
 // Device-to-device copy, one 8-byte element per thread over a grid that
-// covers the array once: on MI355X 6.4 TB/s on 2 GiB, where hipMemcpy's copy
-// (a grid-stride kernel) reaches 5.0 TB/s.
+// covers the array once: on MI355X 5.8 TB/s (read + write) on 2 GiB of
+// nonzero data, where hipMemcpy's device copy reaches 4.9 TB/s and grid-
+// stride loops of 8- or 16-byte elements 4.4-5.7 TB/s (all of them 0.3-0.5
+// TB/s faster on zero-filled buffers).
 __global__ __launch_bounds__(256) void k_stream_copy(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
                                                      size_t n) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -143,8 +145,6 @@ __global__ void k_fill(T* p, size_t n, T v) {
 // set-up kernels
 // ---------------------------------------------------------------------------
 
-// Link mask of every site (see lb_lattice.h). `vel` (wall velocities) may be
-// null, then no site is flagged MOVING.
 // Link mask of every site (see lb_lattice.h) and the class of every wave of
 // 64 sites: 0 when all its sites are fluid with a bare M_FLUID mask (no
 // bounce-back, halo-crossing, Zou-He or moving-wall link), 1 otherwise.

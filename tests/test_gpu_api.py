@@ -206,3 +206,21 @@ def test_fp16_guard_zou_he_inlet_from_rest():
     l2 = float(np.sqrt(np.sum((v16 - v64) ** 2) / np.sum(v64 ** 2)))
     print(f"FP16 inlet: rel. L2 velocity error against FP64 {l2:.3e}", flush=True)
     assert l2 <= 3e-2
+
+
+@pytest.mark.parametrize("nbytes, offset", [(8, 0), (2 ** 20 + 8 * 37, 0), (3 * 2 ** 27, 0), (1000, 3), (7, 8)])
+def test_stream_copy_matches_source(nbytes, offset):
+    """lb_stream_copy (the engine's device copy for checkpoints and
+    snapshots): whole 8-byte elements by its own kernel, any other size or
+    alignment through hipMemcpyAsync; bytes outside the range stay as they
+    were."""
+    import torch
+    from jyulb import _native
+    n = nbytes + offset + 64
+    src = torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.full((n,), 0xAB, dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    _native.stream_copy(dst.data_ptr() + offset, src.data_ptr() + offset, nbytes, st)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[offset:offset + nbytes], src[offset:offset + nbytes])
+    assert bool((dst[:offset] == 0xAB).all()) and bool((dst[offset + nbytes:] == 0xAB).all())

@@ -163,6 +163,7 @@ constexpr uint32_t XCD_CONTIGUOUS = 0xffffffffu;
 
 struct StepParams {
     void* f;
+    void* f2;  // sparse layout: destination of a pull step (k_step_sp_pull), null otherwise
     const uint32_t* mask;
     double* den;
     double* vel;  // 3 * nq, SoA

@@ -656,6 +656,8 @@ struct lb_solver {
     uint32_t* tab = nullptr;
     uint32_t* tabc = nullptr;  // SP_WORDS words per compact site (k_sparse_pack)
     uint32_t* tabw = nullptr;
+    void* f2 = nullptr;   // sparse layout: the second population array of the pull steps (run_steps), or null
+    int sp_pull = -1;     // sparse pull steps: 1 on, 0 off, -1 by sp_pull_on (LB_SP_PULL sets 1 or 0)
     uint64_t sp_kinds[4] = {0, 0, 0, 0};  // link codes by kind (lb_sparse_codes)
     uint32_t nfp = 0;
     uint8_t* phase = nullptr;
@@ -990,6 +992,21 @@ __global__ void k_sparse_tab(StepParams q, const uint32_t* cidx, uint32_t nfp, u
     }
 }
 
+// Whether the sparse layout steps by pull steps (run_steps), given the bytes
+// of one population array. Pull steps against odd AA steps in place (ms per
+// step, sparse porous media of 20% solid): FP64 0.0295 / 0.0254 at
+// 128x64x64, 0.197 / 0.186 at 256x128x128, 0.650 / 0.686 at 384x192x192,
+// 1.545 / 1.587 at 512x256x256, 5.28 / 5.46 at 768x384x384; FP32 2.76 /
+// 2.83 and FP32 storage 2.77 / 2.80 at 768x384x384, within 1% either way
+// from 384x192x192 to 512x256x256; FP16 1.59 / 1.57 at 768x384x384. Below
+// about 1 GiB of populations the lattice sits largely in the 256 MiB last-level
+// cache and the even AA step's purely local access wins; FP16 keeps the AA
+// steps (and half the population memory).
+static bool sp_pull_on(const lb_solver* s, size_t array_bytes) {
+    if (s->sp_pull >= 0) return s->sp_pull != 0;
+    return s->prec != LB_FP16_STORAGE && array_bytes >= ((size_t)1 << 30);
+}
+
 // Compact form of the odd-step table, one wave per 64 compact sites (see
 // sparse_locs). Per link, over the wave's lanes that do not bounce back, the
 // base is the value of loc - lane (modulo 2^32) whose window [base, base +
@@ -1173,6 +1190,8 @@ static void layout_defaults(lb_solver* s, int layout) {
     s->xcd_chunk = sparse ? chunk[s->prec] : 0u;
     s->even_rr = sparse && s->prec == LB_FP64;
     if (const char* env = getenv("LB_EVEN_RR")) s->even_rr = strtol(env, nullptr, 10) != 0;
+    s->sp_pull = -1;  // decided by sp_pull_on at lb_finalize
+    if (const char* env = getenv("LB_SP_PULL")) s->sp_pull = strtol(env, nullptr, 10) != 0 ? 1 : 0;
     if (const char* env = getenv("LB_XCD_CHUNK")) {
         s->xcd_chunk = (uint32_t)strtoul(env, nullptr, 10);
         s->tune_xcd = false;
@@ -1328,7 +1347,7 @@ int lb_destroy(lb_solver* s) {
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
                     s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
                     s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab,
-                    s->tabc, s->tabw, s->dseed, s->st_phase, s->st_den, s->st_vel, s->st_acc};
+                    s->tabc, s->tabw, s->f2, s->dseed, s->st_phase, s->st_den, s->st_vel, s->st_acc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {
@@ -1437,11 +1456,13 @@ static int build_sparse(lb_solver* s) {
         dfree(s, (void**)&s->tab, (Q - 1) * o * 4);
         dfree(s, (void**)&s->tabc, SP_WORDS * o * 4);
         dfree(s, (void**)&s->tabw, SP_WAVE_WORDS * (o / 64) * 4);
+        dfree(s, &s->f2, Q * o * ts);
         s->nfp = 0;
         if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tabc, SP_WORDS * nfp * 4)) ||
-            (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)))
+            (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)) ||
+            (sp_pull_on(s, Q * nfp * ts) && (rc = dalloc(s, &s->f2, Q * nfp * ts))))
             return rc;
         s->nfp = (uint32_t)nfp;
     }
@@ -1450,6 +1471,7 @@ static int build_sparse(lb_solver* s) {
     s->rs = 0;
     s->fsize = (int64_t)Q * nfp;
     HIPCHK(hipMemsetAsync(s->f, 0, Q * nfp * ts, s->stream));  // padding sites stay zero
+    if (s->f2) HIPCHK(hipMemsetAsync(s->f2, 0, Q * nfp * ts, s->stream));
     HIPCHK(hipMemsetAsync(s->site, 0, nfp * 4, s->stream));
     // fluid sites per wave -> exclusive scan on the host -> compact indices
     const uint32_t nw = s->nq / 64;
@@ -1635,6 +1657,9 @@ static int fp16_guard(lb_solver* s) {
 // `n` steps with the solver's current launch configuration; the last one
 // stores den/vel if `store_last`. FP16 storage runs its guard every
 // f16_check steps.
+// In the sparse layout, odd steps are pull steps (k_step_sp_pull) when the
+// second array is there: each leaves the state an even step leaves, in the
+// other array, which then becomes s->f; the parity stays odd.
 static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
     StepParams p = params(s);
     for (uint32_t t = 0; t < n; ++t) {
@@ -1644,8 +1669,16 @@ static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
             p = params(s);
         }
         p.store_dv = (store_last && t + 1 == n) ? 1 : 0;
-        launch_any_step(s, p, s->next_odd != 0);
-        s->next_odd ^= 1;
+        if (s->next_odd && s->f2) {
+            p.f2 = s->f2;
+            launch_any_step(s, p, true);
+            std::swap(s->f, s->f2);
+            p.f = s->f;
+            p.f2 = nullptr;
+        } else {
+            launch_any_step(s, p, s->next_odd != 0);
+            s->next_odd ^= 1;
+        }
         s->steps++;
         s->f16_since++;
     }

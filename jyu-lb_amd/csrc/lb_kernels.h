@@ -401,8 +401,9 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
     }
 }
 
-// Step of compact site n; loc: the odd step's locations (sparse_locs)
-template <int P, int RLX, bool ODD, bool FULL, bool DISS = false>
+// Step of compact site n; loc: the odd step's locations (sparse_locs). PULL
+// (with ODD): the odd step's loads, and the even step's stores into p.f2.
+template <int P, int RLX, bool ODD, bool FULL, bool DISS = false, bool PULL = false>
 LBD void step_sparse_at(const StepParams& p, const uint32_t n, const uint32_t* loc) {
     using TS = typename Var<P>::TS;
     using TC = typename Var<P>::TC;
@@ -464,6 +465,10 @@ LBD void step_sparse_at(const StepParams& p, const uint32_t n, const uint32_t* l
     if constexpr (!ODD) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)), f[l], p.scale);
+    } else if constexpr (PULL) {
+        TS* G = (TS*)p.f2;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P, NTSE>(uniform_ptr(G + (int64_t)opp(l) * ps) + n, f[l], p.scale);
     } else {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSO>(l == REST ? own(l) : F + loc[l], f[opp(l)], p.scale);
@@ -484,6 +489,26 @@ __global__ __launch_bounds__(STEP_TILE_MAX) void k_step_sp(StepParams p) {
     const uint32_t n = tile * p.tile + threadIdx.x;
     if (n >= p.nf) return;
     step_sparse<P, RLX, ODD, FULL>(p, n);
+}
+
+// The sparse layout's pull step: from the state an even step leaves (site y's
+// post-collision population l in its own slot opp(l)) it gathers like the
+// odd step, and stores like the even step, into the other population array
+// p.f2, which then holds the next state in the same form. The odd step's
+// stores go back to the gathered locations, which the upstream rows'
+// compact indices place anywhere within a 128-byte line; partial-line stores
+// measured a quarter slower than aligned ones, while misaligned loads cost
+// nothing. The pull step's stores are all aligned, at the price of a second
+// population array.
+template <int P, int RLX, bool FULL>
+__global__ __launch_bounds__(STEP_TILE_MAX) void k_step_sp_pull(StepParams p) {
+    const uint32_t tile = tile_index(p);
+    if (tile >= p.ntiles) return;
+    const uint32_t n = tile * p.tile + threadIdx.x;
+    if (n >= p.nf) return;
+    uint32_t loc[Q];
+    sparse_locs(p, n, loc);
+    step_sparse_at<P, RLX, true, FULL, false, true>(p, n, loc);
 }
 
 // the dissipation objective's gather in the sparse layout (see k_diss)
@@ -797,7 +822,10 @@ template <int P, int RLX>
 static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
     const unsigned grid = tile_grid(p);
     if (p.tab) {  // sparse layout: one kernel, with the mask only where needed
-        if (p.moving || p.nzh) {
+        if (odd && p.f2) {
+            if (p.moving || p.nzh) k_step_sp_pull<P, RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
+            else k_step_sp_pull<P, RLX, false><<<grid, p.tile, p.lds[1], st>>>(p);
+        } else if (p.moving || p.nzh) {
             if (odd) k_step_sp<P, RLX, true, true><<<grid, p.tile, p.lds[1], st>>>(p);
             else k_step_sp<P, RLX, false, true><<<grid, p.tile, p.lds[0], st>>>(p);
         } else {

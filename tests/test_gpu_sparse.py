@@ -121,6 +121,35 @@ def test_sparse_code_kinds():
     assert k["near"] > 0.9 * 18 * nf and k["far"] < 0.02 * 18 * nf
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp16", "fp32_storage"])
+def test_sparse_pull_steps_equal_in_place_steps(precision, monkeypatch):
+    """The sparse layout's pull steps (a second population array, aligned
+    stores) against its odd AA steps in place, bit for bit, over evolve calls
+    of odd and even lengths."""
+    from jyulb.isothermal import Solver
+    lat, steps = (72, 40, 90), (7, 40, 53)
+    out = []
+    for pull in ("1", "0"):
+        monkeypatch.setenv("LB_SP_PULL", pull)
+        print(f"pull {pull} {precision}", flush=True)
+        a, ph = _porous_run(lambda *x, **k: Solver(*x, layout="sparse", **k), lat, steps, precision=precision)
+        out.append(a)
+    for x, y in zip(out[0], out[1]):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("name", ["trt_symwall", "reg_zouhe_x", "bgk_zouhe_neg", "ex_couette", "trt_long"])
+def test_sparse_pull_steps_on_reference_scenarios(name, monkeypatch):
+    """Pull steps forced on small lattices (walls, symmetric faces, Zou-He
+    faces, moving walls) against the rows layout, FP64, bit for bit."""
+    monkeypatch.setenv("LB_SP_PULL", "1")
+    print(f"pull scenario {name}", flush=True)
+    a = _run_fixture(name, layout="rows")
+    b = _run_fixture(name, layout="sparse")
+    for k in a:
+        np.testing.assert_array_equal(b[k], a[k], err_msg=f"{name} {k}")
+
+
 def test_sparse_porous_medium_matches_oracle():
     import jyulb_oracle as jo
     from jyulb.isothermal import Solver
@@ -183,9 +212,10 @@ def test_sparse_device_memory_follows_fluid_sites():
     b_sp = sp.engine.info()["device_bytes"]
     nfp = (nf + 63) // 64 * 64
     ncell = 64 * 32 * 64
-    # rows: 19 f64 per cell; sparse: per padded fluid site 19 f64, its site
-    # index, the 18 u32 locations and their 5-bit codes (three u32), per wave
-    # of 64 the 18 u32 bases and 18 alternates, and a compact index per cell
+    # rows: 19 f64 per cell; sparse: per padded fluid site 19 f64 (one array:
+    # pull steps only from 1 GiB of populations), its site index, the 18 u32
+    # locations and their 5-bit codes (three u32), per wave of 64 the 18 u32
+    # bases and 18 alternates, and a compact index per cell
     sparse = (19 * 8 + 4 + 18 * 4 + 12) * nfp + 36 * 4 * (nfp // 64) + 4 * ncell
     assert b_rows - b_sp == 19 * 8 * ncell - sparse
 

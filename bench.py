@@ -44,11 +44,13 @@ C5_NAME = ("porous medium 768x384x384 (113M cells; spheres r 6..12, 20% solid, s
            "(magic 3/16), tau=0.6, gx=1e-6, periodic")
 
 
-def step_kernel(prec, layout, rlx="1"):
-    """rocprof name pattern of the bulk step kernel (both parities)."""
+def step_kernel(prec, layout, rlx="1", pull=False):
+    """rocprof name pattern of the bulk step kernel (both parities; with the
+    sparse layout's pull steps, the one kernel of every step after the
+    first of a call)."""
     p = {"fp64": 0, "fp32": 1, "fp16": 2, "fp32_storage": 3}[prec]
     if layout == "sparse":
-        return f"k_step_sp<{p},{rlx},*,false>"
+        return f"k_step_sp_pull<{p},{rlx},false>" if pull else f"k_step_sp<{p},{rlx},*,false>"
     if prec == "fp16" and rlx == "1":
         return "k_step_h<1,*>"
     return f"k_step<{p},{rlx},*>"
@@ -208,7 +210,8 @@ def headline(precs, steps, warmup, layout, solid):
         gbs = BYTES_PER_SITE[prec] * nfl * steps / t / 1e9
         res[prec] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
                          ms_per_step=round(1e3 * t / steps, 5), achieved_gbs=round(gbs, 1),
-                         frac=round(gbs / PEAK_HBM_GBS, 4), layout=lay, kernel=step_kernel(prec, lay),
+                         frac=round(gbs / PEAK_HBM_GBS, 4), layout=lay,
+                         kernel=step_kernel(prec, lay, pull=s.engine.sparse_pull()), pull=s.engine.sparse_pull(),
                          nfluid=nfl, device_bytes=int(s.engine.info()["device_bytes"]),
                          fp16_scale=s.engine.info()["fp16_scale"])
         if lay == "sparse":

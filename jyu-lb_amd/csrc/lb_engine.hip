@@ -2414,6 +2414,12 @@ int lb_sparse_codes(lb_solver* s, uint64_t* counts) {
     return LB_OK;
 }
 
+int lb_sparse_pull(lb_solver* s, int* on) {
+    if (!s || !on) return fail(LB_EINVAL, "null argument");
+    *on = s->f2 ? 1 : 0;
+    return LB_OK;
+}
+
 int lb_layout(lb_solver* s, int* layout) {
     if (!s || !layout) return fail(LB_EINVAL, "null argument");
     *layout = s->layout;

@@ -92,6 +92,7 @@ def lib():
         "lb_info": ([H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int),
                      ctypes.POINTER(ctypes.c_uint64), dp], ctypes.c_int),
         "lb_sparse_codes": ([H, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "lb_sparse_pull": ([H, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "lb_synchronize": ([H], ctypes.c_int),
         "lb_set_face_data_device": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "lb_objective": ([H, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
@@ -322,6 +323,12 @@ class Engine:
         c = (ctypes.c_uint64 * 4)()
         self.call("lb_sparse_codes", c)
         return dict(zip(("near", "alt", "bounce", "far"), (int(x) for x in c)))
+
+    def sparse_pull(self):
+        """True when the sparse layout steps by pull steps (a second population array)."""
+        v = ctypes.c_int()
+        self.call("lb_sparse_pull", ctypes.byref(v))
+        return bool(v.value)
 
     def layout(self):
         """Name of the population layout in use ("auto" until finalize resolves it)."""

@@ -52,8 +52,8 @@ def step_kernel(prec, layout, rlx="1", pull=False):
     if layout == "sparse":
         return f"k_step_sp_pull<{p},{rlx},false>" if pull else f"k_step_sp<{p},{rlx},*,false>"
     if prec == "fp16" and rlx == "1":
-        return "k_step_h<1,*>"
-    return f"k_step<{p},{rlx},*>"
+        return "k_step_h<1,true,true>" if pull else "k_step_h<1,*>"
+    return f"k_step<{p},{rlx},true,true>" if pull else f"k_step<{p},{rlx},*>"
 
 
 # ---------------------------------------------------------------------------
@@ -211,7 +211,7 @@ def headline(precs, steps, warmup, layout, solid):
         res[prec] = dict(mlups=round(nfl * steps / t / 1e6, 1), mlups_cells=round(ncell * steps / t / 1e6, 1),
                          ms_per_step=round(1e3 * t / steps, 5), achieved_gbs=round(gbs, 1),
                          frac=round(gbs / PEAK_HBM_GBS, 4), layout=lay,
-                         kernel=step_kernel(prec, lay, pull=s.engine.sparse_pull()), pull=s.engine.sparse_pull(),
+                         kernel=step_kernel(prec, lay, pull=s.engine.pull_steps()), pull=s.engine.pull_steps(),
                          nfluid=nfl, device_bytes=int(s.engine.info()["device_bytes"]),
                          fp16_scale=s.engine.info()["fp16_scale"])
         if lay == "sparse":
@@ -242,7 +242,7 @@ def cavity_workload(n, steps, warmup, precs):
         gbs = BYTES_PER_SITE[prec] * nfl * steps / t / 1e9
         out[prec] = dict(mlups=round(nfl * steps / t / 1e6, 1), ms_per_step=round(1e3 * t / steps, 5),
                          achieved_gbs=round(gbs, 1), frac=round(gbs / PEAK_HBM_GBS, 4), layout=lay,
-                         kernel=step_kernel(prec, lay))
+                         kernel=step_kernel(prec, lay, pull=s.engine.pull_steps()), pull=s.engine.pull_steps())
         print(f"[bench] cavity {n}^3 {prec} {lay}: {out[prec]['mlups']} MLUPS", file=sys.stderr, flush=True)
         del s
         torch.cuda.synchronize()

@@ -268,12 +268,12 @@ int lb_info(lb_solver *s, uint64_t *steps, int *next_odd, uint64_t *device_bytes
  * the solver is finalized in the sparse layout. */
 int lb_sparse_codes(lb_solver *s, uint64_t counts[4]);
 
-/* Whether the sparse layout steps by pull steps (no counterpart in the
- * reference): a second population array into which each odd step stores
- * line-aligned, chosen at lb_finalize from 1 GiB of populations (FP64, FP32
- * and FP32 storage; LB_SP_PULL=0/1 in the environment decides instead).
- * Results are identical either way. *on = 0 for the dense layouts. */
-int lb_sparse_pull(lb_solver *s, int *on);
+/* Whether the solver steps by pull steps (no counterpart in the reference):
+ * a second population array into which each odd step stores line-aligned.
+ * The sparse layout takes them from 1 GiB of populations in FP64, FP32 and
+ * FP32 storage; LB_SP_PULL=0/1 in the environment decides instead, also for
+ * the dense layouts. Results are identical either way. */
+int lb_pull_steps(lb_solver *s, int *on);
 
 /* The population layout in use (LB_LAYOUT_*; LB_LAYOUT_AUTO until the first
  * lb_finalize resolves it). */

@@ -657,7 +657,8 @@ struct lb_solver {
     uint32_t* tabc = nullptr;  // SP_WORDS words per compact site (k_sparse_pack)
     uint32_t* tabw = nullptr;
     void* f2 = nullptr;   // sparse layout: the second population array of the pull steps (run_steps), or null
-    int sp_pull = -1;     // sparse pull steps: 1 on, 0 off, -1 by sp_pull_on (LB_SP_PULL sets 1 or 0)
+    int sp_pull = -1;     // pull steps: 1 on, 0 off, -1 by sp_pull_on / the launch tuner (LB_SP_PULL sets 1 or 0)
+    bool pull_active = false;  // run_steps takes pull steps (f2 allocated); a dense f2 may be a tuning candidate
     uint64_t sp_kinds[4] = {0, 0, 0, 0};  // link codes by kind (lb_sparse_codes)
     uint32_t nfp = 0;
     uint8_t* phase = nullptr;
@@ -1243,10 +1244,24 @@ static int layout_arrays(lb_solver* s, int layout) {
         s->ps = s->d.nzp;
         s->fsize = rows * s->rs;
     }
-    int rc = dalloc(s, &s->f, (size_t)s->fsize * ts_size(s->prec));
+    const size_t fb = (size_t)s->fsize * ts_size(s->prec);
+    int rc = dalloc(s, &s->f, fb);
     if (rc) return rc;
-    HIPCHK(hipMemset(s->f, 0, (size_t)s->fsize * ts_size(s->prec)));
+    HIPCHK(hipMemset(s->f, 0, fb));
+    // Dense layouts: pull steps when asked for (LB_SP_PULL=1), or as the
+    // launch tuner's last candidate (tune_step), which frees the second
+    // array again if the in-place steps stay faster.
+    if (s->sp_pull == 1 || (s->sp_pull < 0 && s->tune)) {
+        if ((rc = dalloc(s, &s->f2, fb))) return rc;
+        HIPCHK(hipMemset(s->f2, 0, fb));
+        s->pull_active = s->sp_pull == 1;
+    }
     return LB_OK;
+}
+
+// The second array of a dense pull-step candidate that was not chosen.
+static void drop_pull_candidate(lb_solver* s) {
+    if (s->f2 && !s->pull_active) dfree(s, &s->f2, (size_t)s->fsize * ts_size(s->prec));
 }
 
 int lb_create(const lb_config* cfg, lb_solver** out) {
@@ -1464,6 +1479,7 @@ static int build_sparse(lb_solver* s) {
             (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)) ||
             (sp_pull_on(s, Q * nfp * ts) && (rc = dalloc(s, &s->f2, Q * nfp * ts))))
             return rc;
+        s->pull_active = s->f2 != nullptr;
         s->nfp = (uint32_t)nfp;
     }
     if (!s->cidx && (rc = dalloc(s, (void**)&s->cidx, (size_t)s->nq * 4))) return rc;
@@ -1669,7 +1685,7 @@ static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
             p = params(s);
         }
         p.store_dv = (store_last && t + 1 == n) ? 1 : 0;
-        if (s->next_odd && s->f2) {
+        if (s->next_odd && s->pull_active) {
             p.f2 = s->f2;
             launch_any_step(s, p, true);
             std::swap(s->f, s->f2);
@@ -1737,11 +1753,55 @@ static int tune_candidates(const lb_solver* s, StepCfg* c) {
     return n;
 }
 
+// The dense layouts' pull steps as the tuner's last candidate, with the
+// chosen launch: the in-place steps and then the pull steps, TUNE_STEPS each;
+// the pull steps stay if more than TUNE_MARGIN faster. Measured with the
+// tuning off (ms per step, pull / in place): 256^3 cavity FP64 0.929 / 0.956,
+// FP16 0.284 / 0.298, FP32 0.515 / 0.493, FP32 storage 0.507 / 0.515;
+// 512x128x128 duct FP64 0.465 / 0.452, FP16 0.152 / 0.159; the dense
+// 768x384x384 medium FP64 7.06 / 7.48, FP16 2.22 / 2.48.
+constexpr int TUNE_PULL_STEPS = 2 * TUNE_STEPS + 1;
+static int tune_pull(lb_solver* s) {
+    if (!s->f2 || s->pull_active) return LB_OK;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ok = true;
+    for (hipEvent_t& e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    int rc = LB_OK;
+    float ms_aa = 0.0f, ms_pull = 0.0f;
+    if (ok) {
+        HIPCHK(hipEventRecord(ev[0], s->stream));
+        rc = run_steps(s, TUNE_STEPS, false);
+        if (!rc) {
+            HIPCHK(hipEventRecord(ev[1], s->stream));
+            s->pull_active = true;
+            rc = run_steps(s, 1, false);  // reaches the state pull steps start from
+        }
+        if (!rc) {
+            HIPCHK(hipEventRecord(ev[2], s->stream));
+            rc = run_steps(s, TUNE_STEPS, false);
+        }
+        if (!rc) {
+            HIPCHK(hipEventRecord(ev[3], s->stream));
+            HIPCHK(hipEventSynchronize(ev[3]));
+            HIPCHK(hipEventElapsedTime(&ms_aa, ev[0], ev[1]));
+            HIPCHK(hipEventElapsedTime(&ms_pull, ev[2], ev[3]));
+            if (s->tune_verbose)
+                fprintf(stderr, "[jyulb] in-place steps %.4f ms, pull steps %.4f ms per step\n", ms_aa / TUNE_STEPS,
+                        ms_pull / TUNE_STEPS);
+        }
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    s->pull_active = ok && !rc && ms_pull < (1.0f - TUNE_MARGIN) * ms_aa;
+    if (!s->sparse) drop_pull_candidate(s);  // the state is in s->f either way
+    return rc;
+}
+
 static int tune_step(lb_solver* s) {
     StepCfg c[TUNE_MAX];
     const int n = tune_candidates(s, c);
     s->tune = false;
-    if (n < 2) return LB_OK;
+    if (n < 2) return tune_pull(s);
     std::vector<hipEvent_t> ev(2 * n, nullptr);
     for (hipEvent_t& e : ev)
         if (hipEventCreate(&e) != hipSuccess) e = nullptr;
@@ -1749,7 +1809,7 @@ static int tune_step(lb_solver* s) {
         if (!e) {
             for (hipEvent_t x : ev)
                 if (x) (void)hipEventDestroy(x);
-            return LB_OK;  // no timing: keep the default
+            return tune_pull(s);  // no timing: keep the default (and the in-place steps)
         }
     int rc = run_steps(s, TUNE_WARM, false);
     for (int i = 0; i < n && !rc; ++i) {
@@ -1794,7 +1854,7 @@ static int tune_step(lb_solver* s) {
     s->step_lds[1] = c[best].lds_odd;
     s->xcd_cells = c[best].xcd_cells;
     s->xcd_chunk = c[best].chunk;
-    return LB_OK;
+    return tune_pull(s);
 }
 
 // The steps of one evolve call: the FP16 guard, the launch tuning of the
@@ -1808,8 +1868,10 @@ static int evolve_steps(lb_solver* s, uint32_t steps) {
         // steps; a shorter call leaves it to a later one
         StepCfg c[TUNE_MAX];
         const int n = tune_candidates(s, c);
-        if (n < 2) s->tune = false;
-        else if (steps > (uint32_t)(TUNE_WARM + n * TUNE_STEPS)) {
+        const int np = s->f2 && !s->pull_active ? TUNE_PULL_STEPS : 0;
+        if (n < 2 && !np) {
+            s->tune = false;
+        } else if (steps > (uint32_t)(TUNE_WARM + n * TUNE_STEPS + np)) {
             const uint64_t before = s->steps;
             if ((rc = tune_step(s))) return rc;
             steps -= (uint32_t)(s->steps - before);
@@ -2414,9 +2476,9 @@ int lb_sparse_codes(lb_solver* s, uint64_t* counts) {
     return LB_OK;
 }
 
-int lb_sparse_pull(lb_solver* s, int* on) {
+int lb_pull_steps(lb_solver* s, int* on) {
     if (!s || !on) return fail(LB_EINVAL, "null argument");
-    *on = s->f2 ? 1 : 0;
+    *on = s->pull_active ? 1 : 0;
     return LB_OK;
 }
 

@@ -113,7 +113,9 @@ LBD void diss_site(const StepParams& p, const uint32_t c, const TC* f, const TC*
         N[0] * N[0] + N[3] * N[3] + N[5] * N[5] + TC(2.0) * (N[1] * N[1] + N[2] * N[2] + N[4] * N[4]);
 }
 
-template <int P, int RLX, bool ODD, int MODE, bool DISS = false>
+// PULL (with ODD): the odd step's loads, and the even step's stores into p.f2
+// (the dense form of the sparse layout's pull step, k_step_sp_pull).
+template <int P, int RLX, bool ODD, int MODE, bool DISS = false, bool PULL = false>
 LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     constexpr bool PLAIN = MODE != MODE_SPECIAL;  // no special handling needed
     using TS = typename Var<P>::TS;
@@ -225,6 +227,10 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     if constexpr (PLAIN && !ODD) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSE>(own(opp(l)) + kk, f[l], p.scale);
+    } else if constexpr (PLAIN && PULL) {
+        TS* const Gr = (TS*)p.f2 + (int64_t)row * p.rs;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P, NTSE>(uniform_ptr(Gr + (int64_t)opp(l) * p.ps) + kk, f[l], p.scale);
     } else if constexpr (PLAIN) {
         auto push = [&](auto wrap) {
             constexpr bool WRAP = decltype(wrap)::value;
@@ -244,6 +250,10 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
     } else if constexpr (!ODD) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSE>(F + e + (int64_t)opp(l) * p.ps, f[l], p.scale);
+    } else if constexpr (PULL) {
+        TS* const G = (TS*)p.f2;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) stf<P, NTSE>(G + e + (int64_t)opp(l) * p.ps, f[l], p.scale);
     } else {
 #pragma unroll
         for (int l = 0; l < Q; ++l) stf<P, NTSO>(F + odd_loc<true>(p, m, e, l, i, j, k), f[opp(l)], p.scale);
@@ -282,7 +292,7 @@ template <bool ODD>
 LBD constexpr uint32_t special_bits() { return ODD ? M_SPECIAL : M_ZH_ANY; }
 
 // Bulk step: every fluid site that is not special for this parity.
-template <int P, int RLX, bool ODD, bool DISS = false>
+template <int P, int RLX, bool ODD, bool DISS = false, bool PULL = false>
 LBD void step_bulk(const StepParams& p) {
     const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
@@ -291,35 +301,35 @@ LBD void step_bulk(const StepParams& p) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
     if (w >= p.nwaves) return;
     if (LB_PLAIN_PATH && p.wclass[w] == 0) {
-        step_site<P, RLX, ODD, MODE_PLAIN, DISS>(p, c, M_FLUID);
+        step_site<P, RLX, ODD, MODE_PLAIN, DISS, PULL>(p, c, M_FLUID);
         return;
     }
     const uint32_t m = p.mask[c];
     if ((m & (M_FLUID | special_bits<ODD>())) != M_FLUID) return;
-    step_site<P, RLX, ODD, MODE_BULK, DISS>(p, c, m);
+    step_site<P, RLX, ODD, MODE_BULK, DISS, PULL>(p, c, m);
 }
 
-template <int P, int RLX, bool ODD>
+template <int P, int RLX, bool ODD, bool PULL = false>
 __global__ __launch_bounds__(STEP_TILE_MAX) LB_OCC void k_step(StepParams p) {
-    step_bulk<P, RLX, ODD>(p);
+    step_bulk<P, RLX, ODD, false, PULL>(p);
 }
 
 // FP16 storage with TRT: the same kernel held to 64 VGPRs (8 waves per SIMD),
 // which it reaches without spills (69 VGPRs otherwise, 7 waves; ~4% slower on
 // the TRT cavity). BGK and the regularized scheme measured 9-10% slower with
 // the hint (128x64x64 porous channel, 512x128x128 duct) and keep k_step.
-template <int RLX, bool ODD>
+template <int RLX, bool ODD, bool PULL = false>
 __global__ __launch_bounds__(STEP_TILE_MAX) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_h(StepParams p) {
-    step_bulk<LB_FP16_STORAGE, RLX, ODD>(p);
+    step_bulk<LB_FP16_STORAGE, RLX, ODD, false, PULL>(p);
 }
 
 // Special sites, one thread per entry of the compact list.
-template <int P, int RLX, bool ODD>
+template <int P, int RLX, bool ODD, bool PULL = false>
 __global__ __launch_bounds__(TILE) void k_step_special(StepParams p) {
     const uint32_t t = blockIdx.x * TILE + threadIdx.x;
     if (t >= (ODD ? p.nspecial : p.nzh)) return;
     const uint32_t c = p.special[t];
-    step_site<P, RLX, ODD, MODE_SPECIAL>(p, c, p.mask[c]);
+    step_site<P, RLX, ODD, MODE_SPECIAL, false, PULL>(p, c, p.mask[c]);
 }
 
 // Dissipation objective of the step about to run (diss_site): the step's
@@ -834,17 +844,23 @@ static void launch_step(const StepParams& p, bool odd, hipStream_t st) {
         }
         return;
     }
+    // a pull step (odd, into p.f2): the bulk and the special sites both read
+    // p.f and write p.f2, so the two launches touch no common location
+    const bool pull = odd && p.f2;
     if constexpr (P == LB_FP16_STORAGE && RLX == TRT) {
-        if (odd) k_step_h<RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
+        if (pull) k_step_h<RLX, true, true><<<grid, p.tile, p.lds[1], st>>>(p);
+        else if (odd) k_step_h<RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
         else k_step_h<RLX, false><<<grid, p.tile, p.lds[0], st>>>(p);
     } else {
-        if (odd) k_step<P, RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
+        if (pull) k_step<P, RLX, true, true><<<grid, p.tile, p.lds[1], st>>>(p);
+        else if (odd) k_step<P, RLX, true><<<grid, p.tile, p.lds[1], st>>>(p);
         else k_step<P, RLX, false><<<grid, p.tile, p.lds[0], st>>>(p);
     }
     const uint32_t ns = odd ? p.nspecial : p.nzh;
     if (ns) {
         const unsigned g2 = (ns + TILE - 1) / TILE;
-        if (odd) k_step_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
+        if (pull) k_step_special<P, RLX, true, true><<<g2, TILE, 0, st>>>(p);
+        else if (odd) k_step_special<P, RLX, true><<<g2, TILE, 0, st>>>(p);
         else k_step_special<P, RLX, false><<<g2, TILE, 0, st>>>(p);
     }
 }

@@ -92,7 +92,7 @@ def lib():
         "lb_info": ([H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int),
                      ctypes.POINTER(ctypes.c_uint64), dp], ctypes.c_int),
         "lb_sparse_codes": ([H, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
-        "lb_sparse_pull": ([H, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "lb_pull_steps": ([H, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "lb_synchronize": ([H], ctypes.c_int),
         "lb_set_face_data_device": ([H, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "lb_objective": ([H, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
@@ -324,10 +324,10 @@ class Engine:
         self.call("lb_sparse_codes", c)
         return dict(zip(("near", "alt", "bounce", "far"), (int(x) for x in c)))
 
-    def sparse_pull(self):
-        """True when the sparse layout steps by pull steps (a second population array)."""
+    def pull_steps(self):
+        """True when the solver steps by pull steps (a second population array)."""
         v = ctypes.c_int()
-        self.call("lb_sparse_pull", ctypes.byref(v))
+        self.call("lb_pull_steps", ctypes.byref(v))
         return bool(v.value)
 
     def layout(self):

@@ -126,6 +126,17 @@ def test_gpu_fp64_bit_exact_with_xcd_tile_order(name, layout, monkeypatch):
     test_gpu_fp64_bit_exact(path, layout)
 
 
+@pytest.mark.parametrize("layout", [0, 1, 3], ids=["rows", "soa", "sparse"])
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
+def test_gpu_fp64_bit_exact_with_pull_steps(path, layout, monkeypatch):
+    """Pull steps (odd steps that gather like the in-place odd step and store
+    into a second array like the even step), forced on for every fixture:
+    still the reference's bits, moving walls, Zou-He faces, periodic and
+    symmetric faces included."""
+    monkeypatch.setenv("LB_SP_PULL", "1")
+    test_gpu_fp64_bit_exact(path, layout)
+
+
 @pytest.mark.parametrize("tile", ["128", "256", "512"])
 @pytest.mark.parametrize("name", ["trt_walls", "reg_zouhe_y", "bgk_symmetric"])
 def test_gpu_fp64_bit_exact_with_step_tile(name, tile, monkeypatch):

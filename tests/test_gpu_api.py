@@ -176,6 +176,30 @@ def test_fp16_guard_gravity_channel_from_rest():
     assert err <= 1e-1
 
 
+@pytest.mark.parametrize("layout", ["rows", "sparse"])
+def test_fp16_guard_with_pull_steps(layout, monkeypatch):
+    """The FP16 guard's rescales act on the current population array, which
+    the pull steps swap every step: forced pull steps give the in-place
+    steps' bits, rescales included."""
+    from jyulb.isothermal import Solver
+    out = []
+    for pull in ("1", "0"):
+        monkeypatch.setenv("LB_SP_PULL", pull)
+        s = Solver(np.array([4, 34, 4], dtype=np.int32), BGK, 0.1, np.array([3.9e-5, 0.0, 0.0]), [PERIODIC] * 6,
+                   precision="fp16", layout=layout)
+        ph = s.get_phase()
+        ph[...] = 1
+        ph[:, 0, :] = 0
+        ph[:, -1, :] = 0
+        s.finalize_init()
+        for _ in range(3):
+            s.evolve(333)
+        assert s.engine.pull_steps() == (pull == "1")
+        out.append((np.array(s.get_vel()), s.engine.info()["fp16_scale"]))
+    assert out[0][1] == out[1][1] < 2.0 ** 23
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+
+
 def test_fp16_guard_zou_he_inlet_from_rest():
     """A Zou-He inlet of 0.05 switched on for a fluid at rest: the scale chosen
     at finalize_init for the fluid at rest (2^23) would overflow on the first

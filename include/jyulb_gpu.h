@@ -167,6 +167,16 @@ int lb_download_den_vel(lb_solver *s, double *den, double *vel);
  * steps, unpacked to f64 [19][nx][ny][nz] (entries at SOLID sites are 0). */
 int lb_download_post_f(lb_solver *s, double *f);
 
+/* The reference's f2 (isothermal.pxd:150-157): after evolve, the
+ * post-collision populations of the step before the last one. The engine
+ * keeps one population array, so it copies the state before the last step of
+ * every evolve call once lb_keep_previous(s, 1) was called (lb_keep_previous(s,
+ * 0) frees the copy); lb_download_previous_post_f unpacks it like
+ * lb_download_post_f. LB_ESTATE if no evolve call kept one since
+ * lb_finalize. */
+int lb_keep_previous(lb_solver *s, int on);
+int lb_download_previous_post_f(lb_solver *s, double *f);
+
 /* FP64 reductions over the fluid sites of the last stored den/vel:
  * out[0] = sum den, out[1..3] = sum vel, out[4..6] = sum den*vel,
  * out[7] = max |vel|. (The quantities of calc_flow_info,

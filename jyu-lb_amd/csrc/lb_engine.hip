@@ -659,6 +659,13 @@ struct lb_solver {
     void* f2 = nullptr;   // sparse layout: the second population array of the pull steps (run_steps), or null
     int sp_pull = -1;     // pull steps: 1 on, 0 off, -1 by sp_pull_on / the launch tuner (LB_SP_PULL sets 1 or 0)
     bool pull_active = false;  // run_steps takes pull steps (f2 allocated); a dense f2 may be a tuning candidate
+    // the state before the last step of the last evolve call, for the
+    // reference's f2 (lb_keep_previous): copy, its parity and FP16 scale
+    void* prev = nullptr;
+    size_t prev_bytes = 0;
+    int prev_odd = 1;
+    double prev_scale = 1.0;
+    bool keep_prev = false, prev_valid = false;
     uint64_t sp_kinds[4] = {0, 0, 0, 0};  // link codes by kind (lb_sparse_codes)
     uint32_t nfp = 0;
     uint8_t* phase = nullptr;
@@ -1362,7 +1369,7 @@ int lb_destroy(lb_solver* s) {
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
                     s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
                     s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab,
-                    s->tabc, s->tabw, s->f2, s->dseed, s->st_phase, s->st_den, s->st_vel, s->st_acc};
+                    s->tabc, s->tabw, s->f2, s->prev, s->dseed, s->st_phase, s->st_den, s->st_vel, s->st_acc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int f = 0; f < 6; ++f) {
@@ -1542,6 +1549,7 @@ int lb_finalize(lb_solver* s, int64_t* nfluid) {
     inputs_changed(s);
     if (!s->phase_set) return fail(LB_ESTATE, "phase field not uploaded");
     s->finalized = false;  // until the populations below are initialised
+    s->prev_valid = false;
     int rc = build_mask(s, true);
     if (rc) return rc;
     int64_t nf, nm;
@@ -1857,6 +1865,28 @@ static int tune_step(lb_solver* s) {
     return tune_pull(s);
 }
 
+// The state before a call's last step, kept for the reference's f2: after
+// evolve the reference's f2 holds the post-collision populations of the
+// step before the last one (its two arrays are swapped every step,
+// isothermal.pyx:1408-1415).
+static int capture_prev(lb_solver* s) {
+    const size_t fb = (size_t)s->fsize * ts_size(s->prec);
+    if (s->prev && s->prev_bytes != fb) {
+        dfree(s, &s->prev, s->prev_bytes);
+        s->prev_bytes = 0;
+    }
+    int rc;
+    if (!s->prev) {
+        if ((rc = dalloc(s, &s->prev, fb))) return rc;
+        s->prev_bytes = fb;
+    }
+    HIPCHK(stream_copy(s->prev, s->f, fb, s->stream));
+    s->prev_odd = s->next_odd;
+    s->prev_scale = s->scale;
+    s->prev_valid = true;
+    return LB_OK;
+}
+
 // The steps of one evolve call: the FP16 guard, the launch tuning of the
 // first long call, then the steps (den/vel stored on the last one).
 static int evolve_steps(lb_solver* s, uint32_t steps) {
@@ -1877,7 +1907,12 @@ static int evolve_steps(lb_solver* s, uint32_t steps) {
             steps -= (uint32_t)(s->steps - before);
         }
     }
-    if ((rc = run_steps(s, steps, true))) return rc;
+    if (s->keep_prev) {
+        if ((rc = run_steps(s, steps - 1, false)) || (rc = capture_prev(s)) || (rc = run_steps(s, 1, true)))
+            return rc;
+    } else if ((rc = run_steps(s, steps, true))) {
+        return rc;
+    }
     HIPCHK(hipGetLastError());
     return LB_OK;
 }
@@ -2135,24 +2170,21 @@ int lb_download_den_vel_device(lb_solver* s, double* den_dev, double* vel_dev) {
     return LB_OK;
 }
 
-int lb_download_post_f(lb_solver* s, double* f) {
-    if (!s || !f) return fail(LB_EINVAL, "null argument");
-    if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
+// Post-collision populations of the state in `fsrc` (parity next_odd, FP16
+// scale `scale`) unpacked to f64 [19][nx][ny][nz] in host memory `f`.
+static int download_post_f(lb_solver* s, void* fsrc, int next_odd, double scale, double* f) {
     double* buf = nullptr;
     int rc = dalloc(s, (void**)&buf, (size_t)Q * s->ncell * 8);
     if (rc) return rc;
     StepParams p = params(s);
+    p.f = fsrc;
+    p.scale = (float)scale;
+    p.inv_scale = (float)(1.0 / scale);
     switch (s->prec) {
-    case LB_FP64: launch_post_f<LB_FP64>(p, s->next_odd, buf, s->stream); break;
-    case LB_FP32_SHIFTED:
-        launch_post_f<LB_FP32_SHIFTED>(p, s->next_odd, buf, s->stream);
-        break;
-    case LB_FP32_STORAGE:
-        launch_post_f<LB_FP32_STORAGE>(p, s->next_odd, buf, s->stream);
-        break;
-    default:
-        launch_post_f<LB_FP16_STORAGE>(p, s->next_odd, buf, s->stream);
-        break;
+    case LB_FP64: launch_post_f<LB_FP64>(p, next_odd, buf, s->stream); break;
+    case LB_FP32_SHIFTED: launch_post_f<LB_FP32_SHIFTED>(p, next_odd, buf, s->stream); break;
+    case LB_FP32_STORAGE: launch_post_f<LB_FP32_STORAGE>(p, next_odd, buf, s->stream); break;
+    default: launch_post_f<LB_FP16_STORAGE>(p, next_odd, buf, s->stream); break;
     }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(f, buf, (size_t)Q * s->ncell * 8, hipMemcpyDeviceToHost, s->stream);
@@ -2161,6 +2193,31 @@ int lb_download_post_f(lb_solver* s, double* f) {
     s->bytes -= (size_t)Q * s->ncell * 8;
     if (e != hipSuccess) return fail(LB_EHIP, "lb_download_post_f: %s", hipGetErrorString(e));
     return LB_OK;
+}
+
+int lb_download_post_f(lb_solver* s, double* f) {
+    if (!s || !f) return fail(LB_EINVAL, "null argument");
+    if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
+    return download_post_f(s, s->f, s->next_odd, s->scale, f);
+}
+
+int lb_keep_previous(lb_solver* s, int on) {
+    if (!s) return fail(LB_EINVAL, "null handle");
+    s->keep_prev = on != 0;
+    if (!s->keep_prev) {
+        dfree(s, &s->prev, s->prev_bytes);
+        s->prev_bytes = 0;
+        s->prev_valid = false;
+    }
+    return LB_OK;
+}
+
+int lb_download_previous_post_f(lb_solver* s, double* f) {
+    if (!s || !f) return fail(LB_EINVAL, "null argument");
+    if (!s->finalized) return fail(LB_ESTATE, "solver not finalized");
+    if (!s->prev_valid)
+        return fail(LB_ESTATE, "no previous step kept: lb_keep_previous before an evolve call");
+    return download_post_f(s, s->prev, s->prev_odd, s->prev_scale, f);
 }
 
 int lb_reduce(lb_solver* s, double out[8]) {

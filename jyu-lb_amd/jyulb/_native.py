@@ -85,6 +85,8 @@ def lib():
         "lb_refresh_walls": ([H], ctypes.c_int),
         "lb_download_den_vel": ([H, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "lb_download_post_f": ([H, ctypes.c_void_p], ctypes.c_int),
+        "lb_keep_previous": ([H, ctypes.c_int], ctypes.c_int),
+        "lb_download_previous_post_f": ([H, ctypes.c_void_p], ctypes.c_int),
         "lb_reduce": ([H, dp], ctypes.c_int),
         "lb_convergence": ([H, dp], ctypes.c_int),
         "lb_upload_acc_device": ([H, ctypes.c_void_p], ctypes.c_int),
@@ -275,6 +277,19 @@ class Engine:
         nx, ny, nz = self.lat
         f = np.empty((19, nx, ny, nz))
         self.call("lb_download_post_f", ptr(f))
+        return f
+
+    def keep_previous(self, on=True):
+        """Keep the state before the last step of every evolve call (lb_keep_previous)."""
+        self.call("lb_keep_previous", int(bool(on)))
+
+    def download_previous_post_f(self):
+        """Post-collision populations of the step before the last one (f2), or
+        None if no evolve call kept them."""
+        nx, ny, nz = self.lat
+        f = np.empty((19, nx, ny, nz))
+        if lib().lb_download_previous_post_f(self.h, ptr(f)) != LB_OK:
+            return None
         return f
 
     def flow_info(self):

@@ -44,7 +44,7 @@ class Solver:
     # constructors
     # ------------------------------------------------------------------
     def __init__(self, lat_size, rlx_scheme=BGK, kvisc=1.0 / 6.0, g=None, bc=None, *,
-                 precision="fp64", rho0=1.0, fp16_scale=0.0, layout="auto"):
+                 precision="fp64", rho0=1.0, fp16_scale=0.0, layout="auto", keep_f2=False):
         """isothermal.pyx:156-298"""
         self.error_msg = ''
         if lat_size is None:
@@ -74,6 +74,9 @@ class Solver:
         self._eng = _native.Engine((NI - 2, NJ - 2, NK - 2), rlx_scheme, kvisc, self._g,
                                    self._bc, precision=precision, rho0=rho0, fp16_scale=fp16_scale,
                                    layout=layout)
+        self._keep_f2 = bool(keep_f2)
+        if self._keep_f2:
+            self._eng.keep_previous(True)
 
         # flow field, isothermal.pyx:243-248, in pinned host memory that the
         # engine reads and writes in place on every evolve call
@@ -179,7 +182,30 @@ class Solver:
         """Newest post-collision populations in the reference's Bundle-C
         layout (isothermal.pyx:1532-1704), flat 19*NCNT; zeros at halo and
         SOLID sites."""
-        post = self.post_collision_f().reshape(LVECS, -1)
+        return self._bundle_c(self.post_collision_f())
+
+    @property
+    def f2(self):
+        """The reference's f2 (isothermal.pxd:150-157): after evolve, the
+        post-collision populations of the step before the last one, in f1's
+        layout (zeros before the first evolve call). The engine holds one
+        population array, so f2 is kept only for a Solver created with
+        keep_f2=True, which copies the state before the last step of every
+        evolve call."""
+        if not self._keep_f2:
+            raise AttributeError("f2 is kept only by a Solver created with keep_f2=True "
+                                 "(the engine holds one population array)")
+        out = np.zeros((LVECS, self.NI, self.NJ, self.NK))
+        if self.init_done and self._fluid is not None:
+            prev = self._eng.download_previous_post_f()
+            if prev is not None:
+                out[:, 1:-1, 1:-1, 1:-1] = prev
+        return self._bundle_c(out)
+
+    def _bundle_c(self, post):
+        """(19, NI, NJ, NK) populations in the Bundle-C order of
+        isothermal.pyx:1536-1704, flat 19*NCNT."""
+        post = post.reshape(LVECS, -1)
         out = np.zeros(LVECS * self.NCNT)
         n = np.arange(self.NCNT)
         for l in range(LVECS):

@@ -248,3 +248,38 @@ def test_stream_copy_matches_source(nbytes, offset):
     torch.cuda.synchronize()
     assert torch.equal(dst[offset:offset + nbytes], src[offset:offset + nbytes])
     assert bool((dst[:offset] == 0xAB).all()) and bool((dst[offset + nbytes:] == 0xAB).all())
+
+
+@pytest.mark.parametrize("pull", ["0", "1"])
+@pytest.mark.parametrize("layout", ["rows", "sparse"])
+def test_f2_holds_the_step_before_the_last(layout, pull, monkeypatch):
+    """The reference's f2 (isothermal.pxd:150-157) after evolve holds the
+    post-collision populations of the step before the last one (its arrays
+    swap every step, isothermal.pyx:1408-1415): with keep_f2=True, f2 after
+    5 + 3 steps equals f1 after 7, bit for bit, with in-place and with pull
+    steps; before the first evolve call f2 is zero; without keep_f2 it is
+    refused."""
+    from jyulb.isothermal import Solver
+    monkeypatch.setenv("LB_SP_PULL", pull)
+    lat = np.array([12, 10, 9], dtype=np.int32)
+    bc = np.array([PERIODIC, PERIODIC, WALL, WALL, SYMMETRIC, SYMMETRIC], dtype=np.int32)
+
+    def make(**kw):
+        s = Solver(lat, TRT, 0.08, np.array([2e-5, 0.0, 1e-6]), bc, layout=layout, **kw)
+        ph = s.get_phase()
+        ph[...] = 1
+        ph[4:7, 3:6, 2:5] = 0
+        s.finalize_init()
+        return s
+
+    a, b = make(keep_f2=True), make()
+    assert not np.any(a.f2)
+    a.evolve(5)
+    a.evolve(3)
+    b.evolve(5)
+    b.evolve(2)
+    np.testing.assert_array_equal(a.f2, b.f1)
+    b.evolve(1)
+    np.testing.assert_array_equal(a.f1, b.f1)
+    with pytest.raises(AttributeError, match="keep_f2=True"):
+        b.f2

@@ -233,11 +233,14 @@ def headline(precs, steps, warmup, layout, solid):
 
 
 def cavity_workload(n, steps, warmup, precs):
+    """The cavity in each precision. The warm-up call is long enough (60
+    steps at least) to hold the launch tuning of a dense lattice, which times
+    up to 8 launch candidates and the pull steps on the call's own steps."""
     import torch
     out = {}
     for prec in precs:
         s, nfl = make_cavity(n, prec)
-        t, _ = time_device(s, steps, warmup)
+        t, _ = time_device(s, steps, max(warmup, 60))
         lay = s.engine.layout()
         gbs = BYTES_PER_SITE[prec] * nfl * steps / t / 1e9
         out[prec] = dict(mlups=round(nfl * steps / t / 1e6, 1), ms_per_step=round(1e3 * t / steps, 5),

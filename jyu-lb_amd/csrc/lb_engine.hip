@@ -738,6 +738,17 @@ static int dalloc(lb_solver* s, void** p, size_t n) {
     return LB_OK;
 }
 
+// An optional array (the second population array of the pull steps): null,
+// and the steps run in place, if the device cannot hold it.
+static void dalloc_optional(lb_solver* s, void** p, size_t n) {
+    if (hipMalloc(p, n) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return;
+    }
+    s->bytes += n;
+}
+
 static void dfree(lb_solver* s, void** p, size_t n) {
     if (!*p) return;
     (void)hipFree(*p);
@@ -1259,9 +1270,9 @@ static int layout_arrays(lb_solver* s, int layout) {
     // launch tuner's last candidate (tune_step), which frees the second
     // array again if the in-place steps stay faster.
     if (s->sp_pull == 1 || (s->sp_pull < 0 && s->tune)) {
-        if ((rc = dalloc(s, &s->f2, fb))) return rc;
-        HIPCHK(hipMemset(s->f2, 0, fb));
-        s->pull_active = s->sp_pull == 1;
+        dalloc_optional(s, &s->f2, fb);
+        if (s->f2) HIPCHK(hipMemset(s->f2, 0, fb));
+        s->pull_active = s->f2 && s->sp_pull == 1;
     }
     return LB_OK;
 }
@@ -1483,9 +1494,9 @@ static int build_sparse(lb_solver* s) {
         if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tabc, SP_WORDS * nfp * 4)) ||
-            (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)) ||
-            (sp_pull_on(s, Q * nfp * ts) && (rc = dalloc(s, &s->f2, Q * nfp * ts))))
+            (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)))
             return rc;
+        if (sp_pull_on(s, Q * nfp * ts)) dalloc_optional(s, &s->f2, Q * nfp * ts);
         s->pull_active = s->f2 != nullptr;
         s->nfp = (uint32_t)nfp;
     }
@@ -1693,7 +1704,7 @@ static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
             p = params(s);
         }
         p.store_dv = (store_last && t + 1 == n) ? 1 : 0;
-        if (s->next_odd && s->pull_active) {
+        if (s->next_odd && s->pull_active && s->f2) {
             p.f2 = s->f2;
             launch_any_step(s, p, true);
             std::swap(s->f, s->f2);

@@ -703,6 +703,9 @@ struct lb_solver {
     void* ckpt = nullptr;  // (nseg + 1) states: segment starts and the final state
     size_t ckpt_bytes = 0;
     void* lam = nullptr;
+    void* lam2 = nullptr;  // sparse layout: the second cotangent array of the pull-form adjoint steps, or null
+    int adj_pull = -1;     // pull-form adjoint steps: 1 on, 0 off, -1 from 1 GiB of cotangents (LB_ADJ_PULL sets 1 or 0)
+    std::vector<uint8_t> ckpt_odd;  // next_odd of each checkpoint
     void* rec = nullptr;
     size_t rec_bytes = 0;
     void* gacc = nullptr;
@@ -1211,6 +1214,8 @@ static void layout_defaults(lb_solver* s, int layout) {
     if (const char* env = getenv("LB_EVEN_RR")) s->even_rr = strtol(env, nullptr, 10) != 0;
     s->sp_pull = -1;  // decided by sp_pull_on at lb_finalize
     if (const char* env = getenv("LB_SP_PULL")) s->sp_pull = strtol(env, nullptr, 10) != 0 ? 1 : 0;
+    s->adj_pull = -1;
+    if (const char* env = getenv("LB_ADJ_PULL")) s->adj_pull = strtol(env, nullptr, 10) != 0 ? 1 : 0;
     if (const char* env = getenv("LB_XCD_CHUNK")) {
         s->xcd_chunk = (uint32_t)strtoul(env, nullptr, 10);
         s->tune_xcd = false;
@@ -1378,7 +1383,7 @@ int lb_destroy(lb_solver* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     else (void)hipDeviceSynchronize();
     void* ptrs[] = {s->f,     s->mask, s->wclass, s->special, s->phase, s->den,  s->vel,  s->acc,
-                    s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->rec,  s->gacc, s->gface,
+                    s->stage, s->part, s->red,    s->uprev,   s->ckpt,  s->lam,  s->lam2, s->rec,  s->gacc, s->gface,
                     s->snap,  s->snap_dv, s->mov_idx, s->ladd_u, s->cidx, s->site, s->tab,
                     s->tabc, s->tabw, s->f2, s->prev, s->dseed, s->st_phase, s->st_den, s->st_vel, s->st_acc};
     for (void* p : ptrs)
@@ -1695,6 +1700,22 @@ static int fp16_guard(lb_solver* s) {
 // In the sparse layout, odd steps are pull steps (k_step_sp_pull) when the
 // second array is there: each leaves the state an even step leaves, in the
 // other array, which then becomes s->f; the parity stays odd.
+// One step from the state in s->f, whose parity is `odd` (updated): a pull
+// step into s->f2, which then becomes s->f, where pull steps are active and
+// the state is in the form an even step leaves; else an in-place AA step.
+static void one_step(lb_solver* s, StepParams& p, int& odd) {
+    if (odd && s->pull_active && s->f2) {
+        p.f2 = s->f2;
+        launch_any_step(s, p, true);
+        std::swap(s->f, s->f2);
+        p.f = s->f;
+        p.f2 = nullptr;
+    } else {
+        launch_any_step(s, p, odd != 0);
+        odd ^= 1;
+    }
+}
+
 static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
     StepParams p = params(s);
     for (uint32_t t = 0; t < n; ++t) {
@@ -1704,16 +1725,7 @@ static int run_steps(lb_solver* s, uint32_t n, bool store_last) {
             p = params(s);
         }
         p.store_dv = (store_last && t + 1 == n) ? 1 : 0;
-        if (s->next_odd && s->pull_active && s->f2) {
-            p.f2 = s->f2;
-            launch_any_step(s, p, true);
-            std::swap(s->f, s->f2);
-            p.f = s->f;
-            p.f2 = nullptr;
-        } else {
-            launch_any_step(s, p, s->next_odd != 0);
-            s->next_odd ^= 1;
-        }
+        one_step(s, p, s->next_odd);
         s->steps++;
         s->f16_since++;
     }
@@ -2337,9 +2349,11 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         else k_fill<float><<<(unsigned)((drows + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->dseed, drows, 0.0f);
     }
     s->adj_po = s->next_odd;
+    s->ckpt_odd.assign(nseg + 1, 1);
     StepParams p = params(s);
     for (uint32_t seg = 0; seg < nseg; ++seg) {
         HIPCHK(stream_copy((char*)s->ckpt + (size_t)seg * fbytes, s->f, fbytes, s->stream));
+        s->ckpt_odd[seg] = (uint8_t)s->next_odd;
         const uint32_t len = std::min(interval, steps - seg * interval);
         for (uint32_t t = 0; t < len; ++t) {
             p.store_dv = (seg * interval + t + 1 == steps) ? 1 : 0;
@@ -2353,13 +2367,13 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
                 if (s->prec == LB_FP64) launch_diss_rlx<LB_FP64>(q, s->cfg.rlx_scheme, s->next_odd != 0, s->stream);
                 else launch_diss_rlx<LB_FP32_SHIFTED>(q, s->cfg.rlx_scheme, s->next_odd != 0, s->stream);
             }
-            launch_any_step(s, p, s->next_odd != 0);
-            s->next_odd ^= 1;
+            one_step(s, p, s->next_odd);
             s->steps++;
         }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(stream_copy((char*)s->ckpt + (size_t)nseg * fbytes, s->f, fbytes, s->stream));
+    s->ckpt_odd[nseg] = (uint8_t)s->next_odd;
     double n = 0.0;
     if (diss) {
         const int nb = 256;
@@ -2404,6 +2418,15 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
     int rc;
     if (!s->lam && (rc = dalloc(s, &s->lam, (size_t)s->fsize * tcs))) return rc;
+    // Pull-form adjoint steps (sparse layout) from 1 GiB of cotangents, like
+    // the forward pull steps: on the 768x384x384 medium the reverse pass took
+    // 5.26 ms per step in FP32 (10.39 in FP64) against 5.58 (10.99) in
+    // place; on the 128x64x64 channel, whose arrays stay in the caches, the
+    // in-place steps are faster (22.5 against 24.3 us per adjoint step).
+    const size_t lbytes = (size_t)s->fsize * tcs;
+    const bool want_lpull = s->tab && (s->adj_pull >= 0 ? s->adj_pull != 0 : lbytes >= ((size_t)1 << 30));
+    if (want_lpull && !s->lam2) dalloc_optional(s, &s->lam2, lbytes);
+    const bool lpull = want_lpull && s->lam2;
     if (!s->gacc && (rc = dalloc(s, &s->gacc, (size_t)3 * s->nq * tcs))) return rc;
     const size_t nface = (size_t)(s->NI > s->NJ ? s->NI : s->NJ) * (s->NJ > s->NK ? s->NJ : s->NK) * 3;
     if (!s->gface && (rc = dalloc(s, (void**)&s->gface, std::max<size_t>(nface, 3 * (size_t)s->NI * s->NJ) * 8)))
@@ -2414,9 +2437,13 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     const unsigned fb = 256;
     if (s->prec == LB_FP64) {
         k_fill<double><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->lam, s->fsize, 0.0);
+        if (lpull)
+            k_fill<double><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->lam2, s->fsize, 0.0);
         k_fill<double><<<(unsigned)((3 * (size_t)s->nq + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->gacc, 3 * (size_t)s->nq, 0.0);
     } else {
         k_fill<float><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->lam, s->fsize, 0.0f);
+        if (lpull)
+            k_fill<float><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->lam2, s->fsize, 0.0f);
         k_fill<float><<<(unsigned)((3 * (size_t)s->nq + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->gacc, 3 * (size_t)s->nq, 0.0f);
     }
     if (fface) k_fill<double><<<(unsigned)((3 * fface + fb - 1) / fb), fb, 0, s->stream>>>(s->gface, 3 * fface, 0.0);
@@ -2427,6 +2454,7 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     p.store_dv = 0;
     AdjParams x;
     x.lam = s->lam;
+    x.lam2 = nullptr;
     x.gacc = s->gacc;
     x.gface = fface ? s->gface : nullptr;
     x.gface_id = gface_id;
@@ -2443,19 +2471,29 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
         } else {
             s->rec_fwd = false;  // rec now holds an earlier segment
             HIPCHK(stream_copy(s->f, (char*)s->ckpt + (size_t)seg * fbytes, fbytes, s->stream));
+            int odd = s->ckpt_odd[seg];  // as the forward pass stepped from this checkpoint
             for (uint32_t t = 0; t < len; ++t) {  // recompute, recording (rho, u*)
                 p.rec = (char*)s->rec + (size_t)t * 4 * s->nq * tcs;
-                launch_any_step(s, p, (s->adj_po ^ ((g0 + t) & 1)) != 0);
+                one_step(s, p, odd);
             }
             p.rec = nullptr;
         }
+        // The adjoint steps need only the record, whichever steps (in place
+        // or pull) the forward pass took: in place, the adjoint of step g is
+        // the adjoint AA step of g's parity in the in-place sequence from
+        // adj_po; in pull form, one kernel, alternating lam and lam2.
         for (int t = (int)len - 1; t >= 0; --t) {
             const uint32_t g = g0 + (uint32_t)t;
             x.rec = (const char*)s->rec + (size_t)t * 4 * s->nq * tcs;
             x.seed = (g + 1 == steps) ? 1 : 0;
             const bool odd = (s->adj_po ^ (g & 1)) != 0;
+            if (lpull) {
+                x.lam = s->lam;
+                x.lam2 = s->lam2;
+            }
             if (s->prec == LB_FP64) launch_adj_rlx<LB_FP64>(p, x, s->cfg.rlx_scheme, odd, s->stream);
             else launch_adj_rlx<LB_FP32_SHIFTED>(p, x, s->cfg.rlx_scheme, odd, s->stream);
+            if (lpull) std::swap(s->lam, s->lam2);
         }
         HIPCHK(hipGetLastError());
     }

@@ -712,6 +712,42 @@ __global__ __launch_bounds__(TILE) void k_adj_sp(StepParams p, AdjParams x) {
     }
 }
 
+// Pull form of the sparse adjoint step, for a step of either parity. The
+// cotangents of the streamed populations are kept in the form an even
+// step's reads use: that of population l at site y in y's own slot l. The
+// post-collision population l of site x streams to site x + c_l, which is
+// the upstream site of link opp(l), so its cotangent is found where the
+// forward odd step reads link opp(l) (loc[opp(l)]; on a bounce-back link
+// that is x's own slot opp(l), where the bounced population's cotangent is).
+// The step gathers those like the odd adjoint step and stores like the even
+// one, into x.lam2, which then holds the previous step's cotangents in the
+// same form. Arithmetic and results are those of k_adj_sp; all stores are
+// aligned, at the price of a second cotangent array.
+template <int P, int RLX, bool ZH>
+__global__ __launch_bounds__(TILE) void k_adj_sp_pull(StepParams p, AdjParams x) {
+    using TC = typename Var<P>::TC;
+    const uint32_t tile = tile_index(p);
+    if (tile >= p.ntiles) return;
+    const uint32_t n = tile * p.tile + threadIdx.x;
+    if (n >= p.nf) return;
+    const TC* L = (const TC*)x.lam;
+    TC* L2 = (TC*)x.lam2;
+    const uint32_t ps = (uint32_t)p.ps;
+    const uint32_t c = p.site[n];
+    const uint32_t m = ZH ? p.mask[c] : M_FLUID;
+    int i = 0, j = 0, k = 0;
+    if ((m & M_ZH_ANY) || x.seed) decompose(c, p.d.ny, p.d.nzp, i, j, k);
+    constexpr bool NTE = (LB_NT_MASK & 1) != 0;
+    uint32_t loc[Q];
+    TC lam[Q], mu[Q];
+    sparse_locs(p, n, loc);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) lam[opp(l)] = L[loc[l]];
+    adj_core<P, RLX>(p, x, c, m, ZH && (m & M_ZH_ANY) != 0, i, j, k, lam, mu);
+#pragma unroll
+    for (int l = 0; l < Q; ++l) st_stream<NTE>(uniform_ptr(L2 + (int64_t)l * ps) + n, mu[l]);
+}
+
 
 // velocity of the site reached from x along +/-c_l for _comp_der_vel:
 // resolved through periodic / symmetric halos, zero for WALL/FIXED halos
@@ -879,7 +915,10 @@ static void launch_adj(const StepParams& ps, const AdjParams& x, bool odd, hipSt
     p.ntiles = ((p.tab ? (uint32_t)p.ps : p.nq) + TILE - 1) / TILE;
     const unsigned grid = tile_grid(p);
     if (p.tab) {  // sparse layout
-        if (p.nzh) {
+        if (x.lam2) {  // pull form, the same kernel at either parity
+            if (p.nzh) k_adj_sp_pull<P, RLX, true><<<grid, TILE, 0, st>>>(p, x);
+            else k_adj_sp_pull<P, RLX, false><<<grid, TILE, 0, st>>>(p, x);
+        } else if (p.nzh) {
             if (odd) k_adj_sp<P, RLX, true, true><<<grid, TILE, 0, st>>>(p, x);
             else k_adj_sp<P, RLX, false, true><<<grid, TILE, 0, st>>>(p, x);
         } else {

@@ -52,7 +52,9 @@ def checkpoint_interval(solver, steps, budget_bytes=None, objective="flux"):
     choice, and i = steps (one segment) recomputes nothing. Memory for
     interval i: (ceil(steps / i) + 1) population arrays (19 values per site)
     for the checkpoints, min(i, steps) x 4 values per site for the (rho, u*)
-    record, one population array of cotangents and 3 values per site of the
+    record, one population array of cotangents (two in the sparse layout,
+    whose adjoint steps from 1 GiB of cotangents write the other array; the
+    budget counts the second array at every size) and 3 values per site of the
     acceleration cotangent, plus (Q + 4) values per site of seeds for the
     dissipation objective; nx * ny * round_up(nz, 64) sites and 8 bytes per
     value in FP64, 4 in FP32. Raises MemoryError if even the smallest
@@ -64,7 +66,8 @@ def checkpoint_interval(solver, steps, budget_bytes=None, objective="flux"):
     nq = int(nx) * int(ny) * ((int(nz) + 63) // 64 * 64)
     vb = 8 if eng.precision == LB_FP64 else 4
     fb, rb = 19 * nq * vb, 4 * nq * vb
-    extra = fb + 3 * nq * vb + (23 * nq * vb if objective == "dissipation" else 0)
+    extra = (2 if eng.layout() == "sparse" else 1) * fb + 3 * nq * vb
+    extra += 23 * nq * vb if objective == "dissipation" else 0
     steps = int(steps)
     if steps < 1:
         raise ValueError('steps must be >= 1')

@@ -117,12 +117,13 @@ def test_checkpoint_interval_choice():
     segments whose memory fits the budget. Lattice 10 x 4 x 70 has
     10 * 4 * 128 = 5120 padded sites: a population array is 778,240 bytes in
     FP64, one step of (rho, u*) record 163,840, and the reverse pass holds a
-    cotangent array and 3 acceleration cotangents per site (901,120 bytes)."""
+    cotangent array and 3 acceleration cotangents per site (901,120 bytes),
+    and in the sparse layout a second cotangent array."""
     from types import SimpleNamespace
     from jyulb.adjoint import checkpoint_interval
     from jyulb._native import LB_FP64, LB_FP32_SHIFTED
-    s64 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64))
-    s32 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP32_SHIFTED))
+    s64 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64, layout=lambda: "rows"))
+    s32 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP32_SHIFTED, layout=lambda: "rows"))
     fb, rb, extra = 778_240, 163_840, 901_120
     one = 2 * fb + 100 * rb + extra  # one segment: 18,841,600 bytes
     assert checkpoint_interval(s64, 100, one) == 100
@@ -143,6 +144,10 @@ def test_checkpoint_interval_choice():
         checkpoint_interval(s64, 100, 6 * fb + 20 * rb + extra - 1)
     with pytest.raises(ValueError):
         checkpoint_interval(s64, 0, 1 << 40)
+    # the sparse layout's adjoint steps hold a second cotangent array
+    sp = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64, layout=lambda: "sparse"))
+    assert checkpoint_interval(sp, 100, one + fb) == 100
+    assert checkpoint_interval(sp, 100, one + fb - 1) == 50
 
 
 @pytest.mark.parametrize("lat", [(1024, 1024, 2048), (1024, 1024, 2049), (4096, 1024, 1024)])

@@ -395,6 +395,44 @@ def test_adjoint_identical_in_sparse_layout(precision, rlx, steps_bc):
     np.testing.assert_array_equal(out[1][2], out[0][2])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("pre_steps", [0, 1], ids=["from_odd", "from_even"])
+def test_adjoint_pull_forms_identical(monkeypatch, precision, pre_steps):
+    """Sparse layout: the forward and recompute passes in place or as pull
+    steps (LB_SP_PULL), and the adjoint steps in place or in pull form
+    (LB_ADJ_PULL, a second cotangent array), from either parity. Objective
+    and both gradients are bit-identical in all four combinations, for the
+    flux objective with segments of unequal length (30 steps, checkpoint
+    every 7) and for the dissipation objective."""
+    from jyulb import adjoint
+    out, diss = {}, {}
+    for sp in ("0", "1"):
+        for ap in ("0", "1"):
+            monkeypatch.setenv("LB_SP_PULL", sp)
+            monkeypatch.setenv("LB_ADJ_PULL", ap)
+            print(f"pull forms {precision} pre {pre_steps} LB_SP_PULL={sp} LB_ADJ_PULL={ap}", flush=True)
+            s, prof, amp = make_channel(TRT, precision, steps_bc=(PERIODIC, PERIODIC), seed=8, layout="sparse")
+            assert s.engine.pull_steps() == (sp == "1")
+            if pre_steps:
+                s.engine.evolve(pre_steps)
+            acc, fden, fvel, g = _inputs(s)
+            s.engine.snapshot()
+            out[sp, ap] = _adjoint(s, acc, fden, fvel, g, 30, 7)
+            s.engine.restore()
+            diss[sp, ap] = adjoint.dissipation_gradient(s, 20, 8, grad_face=FACE_1X)
+    ref = out["0", "0"]
+    assert np.any(ref[1] != 0) and np.any(ref[2] != 0)
+    for key, r in out.items():
+        assert r[0] == ref[0], key
+        np.testing.assert_array_equal(r[1], ref[1], err_msg=str(key))
+        np.testing.assert_array_equal(r[2], ref[2], err_msg=str(key))
+    dref = diss["0", "0"]
+    for key, r in diss.items():
+        assert r[0] == dref[0], key
+        np.testing.assert_array_equal(r[1], dref[1], err_msg=str(key))
+        np.testing.assert_array_equal(r[2], dref[2], err_msg=str(key))
+
+
 def test_adjoint_porous_channel_sparse_equals_rows():
     """Full-size porous channel (128x64x64, FP32, 60 steps, checkpoint every
     25): the sparse layout's gradients equal the dense layout's bit for bit."""

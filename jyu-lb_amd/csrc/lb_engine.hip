@@ -1193,6 +1193,9 @@ int lb_device_count(void) {
     return n;
 }
 
+// Lattices below this many cells are not launch-tuned by default.
+constexpr uint64_t SMALL_SPARSE_CELLS = 1ull << 22;
+
 // Launch defaults of a population layout, then the environment overrides.
 static void layout_defaults(lb_solver* s, int layout) {
     const bool sparse = layout == LB_LAYOUT_SPARSE;
@@ -1210,6 +1213,18 @@ static void layout_defaults(lb_solver* s, int layout) {
     // alone run round-robin.
     const uint32_t chunk[4] = {32, 4, 8, 2};  // by LB_FP64, LB_FP32_SHIFTED, LB_FP16_STORAGE, LB_FP32_STORAGE
     s->xcd_chunk = sparse ? chunk[s->prec] : 0u;
+    // Small sparse lattices, which the launch tuner leaves alone: with 4- and
+    // 2-byte populations contiguous ranges per XCD, whose L2 then holds most
+    // of the upstream rows of its range. Sparse media of 20% solid, ms per
+    // step in contiguous ranges / round-robin: 128x64x64 FP32 0.0135 /
+    // 0.0156, FP16 0.0120 / 0.0135, FP32 storage 0.0149 / 0.0165, FP64
+    // 0.0273 / 0.0265; 256x128x128 FP32 0.0944 / 0.0984, FP16 0.0627 /
+    // 0.0650 (tuned there, with the XCD orders among the candidates);
+    // 384x192x192 FP32 0.359 / 0.345.
+    if (sparse && s->prec != LB_FP64 && s->nq < SMALL_SPARSE_CELLS) {
+        s->xcd_cells = 0;
+        s->xcd_chunk = 0;
+    }
     s->even_rr = sparse && s->prec == LB_FP64;
     if (const char* env = getenv("LB_EVEN_RR")) s->even_rr = strtol(env, nullptr, 10) != 0;
     s->sp_pull = -1;  // decided by sp_pull_on at lb_finalize
@@ -1313,7 +1328,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     // LB_AUTOTUNE_MIN_CELLS changes the size, LB_AUTOTUNE_VERBOSE=1 prints
     // the timings.
     {
-        uint64_t min_cells = 1ull << 22;
+        uint64_t min_cells = SMALL_SPARSE_CELLS;
         if (const char* env = getenv("LB_AUTOTUNE_MIN_CELLS")) min_cells = strtoull(env, nullptr, 10);
         const char* at = getenv("LB_AUTOTUNE");
         s->tune = (!at || strtol(at, nullptr, 10) != 0) && !getenv("LB_STEP_TILE") && !getenv("LB_STEP_LDS_EVEN") &&
@@ -1746,7 +1761,8 @@ struct StepCfg {
 constexpr int TUNE_WARM = 2, TUNE_STEPS = 4;
 constexpr float TUNE_MARGIN = 0.02f;
 
-// the XCD tile order is a candidate too on lattices of at least 2^24 cells
+// the XCD tile orders are candidates too in the sparse layout and on dense
+// lattices of at least 2^24 cells
 constexpr uint64_t TUNE_XCD_CELLS = 1ull << 24;
 constexpr int TUNE_MAX = 10;
 
@@ -1774,7 +1790,7 @@ static int tune_candidates(const lb_solver* s, StepCfg* c) {
             add({256, 26000, 0, xd, ch});
         }
     }
-    if (s->tune_xcd && (uint64_t)s->nq >= TUNE_XCD_CELLS) {  // each of them in the other tile orders
+    if (s->tune_xcd && (s->sparse || (uint64_t)s->nq >= TUNE_XCD_CELLS)) {  // each of them in the other orders
         const int m = n;
         for (int i = 0; i < m; ++i) {
             add({c[i].tile, c[i].lds_even, c[i].lds_odd, xd ? 0 : UINT64_MAX, ch});

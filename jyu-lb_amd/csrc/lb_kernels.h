@@ -266,7 +266,8 @@ LBD void step_site(const StepParams& p, const uint32_t c, const uint32_t m) {
 // lattices of at least LB_XCD_REMAP_CELLS cells (an environment variable of
 // that name, read at lb_create, overrides it): the 768x384x384 porous
 // medium ran 5-7% faster with it, while the 256^3 cavity, the 512x128x128
-// duct and the 128x64x64 porous channel ran 1.5-3.8% faster without.
+// duct and the 128x64x64 porous channel ran 1.5-3.8% faster without in the
+// dense layout (the sparse layout's own defaults: layout_defaults).
 // With p.xcd = C other than XCD_CONTIGUOUS, the XCDs take chunks of C
 // consecutive tiles in turn (chunk m to XCD m % 8) and the grid is padded to
 // a multiple of 8C.

@@ -243,7 +243,9 @@ int lb_objective(lb_solver *s, int face, int axis, double *J);
  * `interval` steps, and returns the objective J: lb_objective(face, axis)
  * after the last step, or with face = LB_OBJ_DISSIPATION the dissipation
  * objective. The last segment also records (rho, u*) per site and step
- * (min(interval, steps) x 4 x nq values of the compute type). The steps are
+ * (min(interval, steps) x 4 rows of the compute type: nq values per row in
+ * the dense layouts, the fluid sites rounded up to 64 in the sparse layout).
+ * The steps are
  * pull steps where lb_evolve's are (lb_pull_steps). FP64 or FP32 precision;
  * stationary walls only. */
 int lb_adj_forward(lb_solver *s, uint32_t steps, uint32_t interval, int face, int axis, double *J);

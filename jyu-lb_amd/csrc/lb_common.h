@@ -193,7 +193,8 @@ struct StepParams {
     // the rest; ascending within each wave
     const uint32_t* special;
     uint32_t nspecial, nzh;
-    void* rec;  // if set: (rho, u*) of this step, 4 * nq of the compute type (adjoint recompute)
+    void* rec;  // if set: (rho, u*) of this step, 4 rows of the compute type (adjoint recompute):
+                // nq values by cell in the dense layouts, ps by compact site in the sparse layout
     // sparse layout (LB_LAYOUT_SPARSE), null otherwise: compact index of every
     // site (fluid sites only), site of every compact index, and the odd-step
     // location of link l of compact site n at tab[link_slot(l) * ps + n]
@@ -289,7 +290,7 @@ struct AdjParams {
     void* lam;          // cotangent array, population layout of f, compute type
     void* lam2;         // sparse layout, pull form: the array the step writes (null: in place)
     const void* rec;    // (rho, u*) recorded by the forward recompute of this step
-    void* gacc;         // accumulated acceleration cotangent, 3 * nq, compute type
+    void* gacc;         // accumulated acceleration cotangent, compute type, 3 rows indexed like rec
     double* gface;      // accumulated face-velocity cotangent (face of gface_id), face layout x 3
     int gface_id;
     int seed;           // 1 on the objective's step

@@ -112,6 +112,18 @@ __global__ void k_export_vec3(const TC* src, double* out, Dims d, uint32_t nq) {
     for (int a = 0; a < 3; ++a) out[3 * o + a] = (double)src[(int64_t)a * nq + c];
 }
 
+// The same from rows indexed by compact site (nf sites, rows of nfp values;
+// site[n] is the cell of compact site n) into an output zeroed beforehand.
+template <class TC>
+__global__ void k_export_vec3_sp(const TC* src, const uint32_t* site, uint32_t nf, uint32_t nfp, double* out, Dims d) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nf) return;
+    int i, j, k;
+    decompose(site[n], d.ny, d.nzp, i, j, k);
+    const int64_t o = ((int64_t)i * d.ny + j) * d.nz + k;
+    for (int a = 0; a < 3; ++a) out[3 * o + a] = (double)src[(int64_t)a * nfp + n];
+}
+
 // Device-to-device copy, one 8-byte element per thread over a grid that
 // covers the array once: on MI355X 5.8 TB/s (read + write) on 2 GiB of
 // nonzero data, where hipMemcpy's device copy reaches 4.9 TB/s and grid-
@@ -730,6 +742,11 @@ static_assert(sizeof(scratch) == sizeof(double) * RED_PARTS * FLOW_N, "scratch h
 
 static size_t ts_size(int prec) { return prec == LB_FP64 ? 8 : prec == LB_FP16_STORAGE ? 2 : 4; }
 static size_t tc_size(int prec) { return prec == LB_FP64 || prec == LB_FP32_STORAGE ? 8 : 4; }
+
+// Values per row of the adjoint's (rho, u*) record and acceleration
+// cotangent: one per cell in the dense layouts, one per compact site in the
+// sparse layout (whose kernels then read no cell index for them).
+static size_t rec_row(const lb_solver* s) { return s->sparse ? (size_t)s->nfp : (size_t)s->nq; }
 
 static int dalloc(lb_solver* s, void** p, size_t n) {
     hipError_t e = hipMalloc(p, n);
@@ -2314,9 +2331,9 @@ int lb_objective(lb_solver* s, int face, int axis, double* J) {
     return objective(s, face, axis, J, nullptr);
 }
 
-// The (rho, u*) record of one checkpoint segment: `len` steps of 4 * nq values.
+// The (rho, u*) record of one checkpoint segment: `len` steps of 4 rows.
 static int ensure_rec(lb_solver* s, uint32_t len) {
-    const size_t recb = (size_t)len * 4 * s->nq * tc_size(s->prec);
+    const size_t recb = (size_t)len * 4 * rec_row(s) * tc_size(s->prec);
     if (s->rec_bytes >= recb) return LB_OK;
     s->rec_fwd = false;
     if (s->rec) {
@@ -2374,7 +2391,7 @@ int lb_adj_forward(lb_solver* s, uint32_t steps, uint32_t interval, int face, in
         for (uint32_t t = 0; t < len; ++t) {
             p.store_dv = (seg * interval + t + 1 == steps) ? 1 : 0;
             // the last segment records (rho, u*) as the reverse pass's recompute would
-            p.rec = seg + 1 == nseg ? (char*)s->rec + (size_t)t * 4 * s->nq * tcs : nullptr;
+            p.rec = seg + 1 == nseg ? (char*)s->rec + (size_t)t * 4 * rec_row(s) * tcs : nullptr;
             if (diss && p.store_dv) {  // the objective is on the final step's collision input
                 StepParams q = p;
                 q.rec = nullptr;
@@ -2443,7 +2460,8 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     const bool want_lpull = s->tab && (s->adj_pull >= 0 ? s->adj_pull != 0 : lbytes >= ((size_t)1 << 30));
     if (want_lpull && !s->lam2) dalloc_optional(s, &s->lam2, lbytes);
     const bool lpull = want_lpull && s->lam2;
-    if (!s->gacc && (rc = dalloc(s, &s->gacc, (size_t)3 * s->nq * tcs))) return rc;
+    const size_t rn = rec_row(s);
+    if (!s->gacc && (rc = dalloc(s, &s->gacc, 3 * rn * tcs))) return rc;
     const size_t nface = (size_t)(s->NI > s->NJ ? s->NI : s->NJ) * (s->NJ > s->NK ? s->NJ : s->NK) * 3;
     if (!s->gface && (rc = dalloc(s, (void**)&s->gface, std::max<size_t>(nface, 3 * (size_t)s->NI * s->NJ) * 8)))
         return rc;
@@ -2455,12 +2473,12 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
         k_fill<double><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->lam, s->fsize, 0.0);
         if (lpull)
             k_fill<double><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->lam2, s->fsize, 0.0);
-        k_fill<double><<<(unsigned)((3 * (size_t)s->nq + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->gacc, 3 * (size_t)s->nq, 0.0);
+        k_fill<double><<<(unsigned)((3 * rn + fb - 1) / fb), fb, 0, s->stream>>>((double*)s->gacc, 3 * rn, 0.0);
     } else {
         k_fill<float><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->lam, s->fsize, 0.0f);
         if (lpull)
             k_fill<float><<<(unsigned)((s->fsize + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->lam2, s->fsize, 0.0f);
-        k_fill<float><<<(unsigned)((3 * (size_t)s->nq + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->gacc, 3 * (size_t)s->nq, 0.0f);
+        k_fill<float><<<(unsigned)((3 * rn + fb - 1) / fb), fb, 0, s->stream>>>((float*)s->gacc, 3 * rn, 0.0f);
     }
     if (fface) k_fill<double><<<(unsigned)((3 * fface + fb - 1) / fb), fb, 0, s->stream>>>(s->gface, 3 * fface, 0.0);
     HIPCHK(hipGetLastError());
@@ -2489,7 +2507,7 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
             HIPCHK(stream_copy(s->f, (char*)s->ckpt + (size_t)seg * fbytes, fbytes, s->stream));
             int odd = s->ckpt_odd[seg];  // as the forward pass stepped from this checkpoint
             for (uint32_t t = 0; t < len; ++t) {  // recompute, recording (rho, u*)
-                p.rec = (char*)s->rec + (size_t)t * 4 * s->nq * tcs;
+                p.rec = (char*)s->rec + (size_t)t * 4 * rec_row(s) * tcs;
                 one_step(s, p, odd);
             }
             p.rec = nullptr;
@@ -2500,7 +2518,7 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
         // adj_po; in pull form, one kernel, alternating lam and lam2.
         for (int t = (int)len - 1; t >= 0; --t) {
             const uint32_t g = g0 + (uint32_t)t;
-            x.rec = (const char*)s->rec + (size_t)t * 4 * s->nq * tcs;
+            x.rec = (const char*)s->rec + (size_t)t * 4 * rec_row(s) * tcs;
             x.seed = (g + 1 == steps) ? 1 : 0;
             const bool odd = (s->adj_po ^ (g & 1)) != 0;
             if (lpull) {
@@ -2517,10 +2535,21 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     HIPCHK(stream_copy(s->f, (char*)s->ckpt + (size_t)nseg * fbytes, fbytes, s->stream));
     if ((rc = ensure_stage(s))) return rc;
     if (grad_acc) {
-        if (s->prec == LB_FP64)
+        if (s->sparse) {  // fluid sites from the compact rows, zero elsewhere
+            const size_t no = (size_t)s->ncell * 3;
+            k_fill<double><<<(unsigned)((no + 255) / 256), 256, 0, s->stream>>>(s->stage, no, 0.0);
+            const uint32_t nf = (uint32_t)s->nfluid;
+            if (nf && s->prec == LB_FP64)
+                k_export_vec3_sp<double><<<blocks_for(nf, 256), 256, 0, s->stream>>>((const double*)s->gacc, s->site, nf,
+                                                                                  s->nfp, s->stage, s->d);
+            else if (nf)
+                k_export_vec3_sp<float><<<blocks_for(nf, 256), 256, 0, s->stream>>>((const float*)s->gacc, s->site, nf,
+                                                                                 s->nfp, s->stage, s->d);
+        } else if (s->prec == LB_FP64) {
             k_export_vec3<double><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>((const double*)s->gacc, s->stage, s->d, s->nq);
-        else
+        } else {
             k_export_vec3<float><<<blocks_for(s->nq, 256), 256, 0, s->stream>>>((const float*)s->gacc, s->stage, s->d, s->nq);
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(grad_acc, s->stage, (size_t)s->ncell * 24,
                               device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s->stream));

@@ -422,7 +422,7 @@ LBD void step_sparse_at(const StepParams& p, const uint32_t n, const uint32_t* l
     TS* F = (TS*)p.f;
     const uint32_t ps = (uint32_t)p.ps;  // 19 * ps < 2^32 (lb_finalize)
     const TC* A = (const TC*)p.acc;
-    const bool need_c = FULL || DISS || A || p.store_dv || p.rec;  // uniform
+    const bool need_c = FULL || DISS || A || p.store_dv;  // uniform
     const uint32_t c = need_c ? p.site[n] : 0u;
     const uint32_t m = FULL ? p.mask[c] : M_FLUID;
     constexpr bool NTLE = (LB_NT_MASK & 1) != 0, NTSE = (LB_NT_MASK & 2) != 0;
@@ -466,12 +466,12 @@ LBD void step_sparse_at(const StepParams& p, const uint32_t n, const uint32_t* l
     } else if (FULL && p.moving && (m & M_MOVING)) {
         p.den[c] = SH ? p.rho0 + (double)ds : (double)den;
     }
-    if (p.rec) {
+    if (p.rec) {  // by compact site: rows of ps values
         TC* R = (TC*)p.rec;
-        R[c] = den;
-        R[nq + c] = vel[0];
-        R[2 * nq + c] = vel[1];
-        R[3 * nq + c] = vel[2];
+        R[n] = den;
+        R[ps + n] = vel[0];
+        R[2 * ps + n] = vel[1];
+        R[3 * ps + n] = vel[2];
     }
     if constexpr (!ODD) {
 #pragma unroll
@@ -546,14 +546,19 @@ __global__ __launch_bounds__(STEP_TILE_MAX) void k_diss_sp(StepParams p) {
 // from the recorded (rho, u*), the objective's seed, the acceleration
 // cotangent and, on Zou-He sites (zh), the faces' VJP in reverse order.
 // lam: cotangents of the post-collision populations; mu: of the streamed ones.
+// c: the site's cell (read only for a per-site acceleration field and the
+// dissipation seeds); r: its index in the record and in x.gacc, rows of rs
+// values (the cell and nq in the dense layouts, the compact site and ps in
+// the sparse layout).
 template <int P, int RLX>
-LBD void adj_core(const StepParams& p, const AdjParams& x, const uint32_t c, const uint32_t m, const bool zh, int i,
-                  int j, int k, const typename Var<P>::TC* lam, typename Var<P>::TC* mu) {
+LBD void adj_core(const StepParams& p, const AdjParams& x, const uint32_t c, const uint32_t r, const int64_t rs,
+                  const uint32_t m, const bool zh, int i, int j, int k, const typename Var<P>::TC* lam,
+                  typename Var<P>::TC* mu) {
     using TC = typename Var<P>::TC;
     const int64_t nq = p.nq;
     const TC* R = (const TC*)x.rec;
-    const TC rho = R[c];
-    const TC us[3] = {R[nq + c], R[2 * nq + c], R[3 * nq + c]};
+    const TC rho = R[r];
+    const TC us[3] = {R[rs + r], R[2 * rs + r], R[3 * rs + r]};
     TC acc[3];
     const TC* A = (const TC*)p.acc;
 #pragma unroll
@@ -583,7 +588,7 @@ LBD void adj_core(const StepParams& p, const AdjParams& x, const uint32_t c, con
     if (x.gacc) {
         TC* G = (TC*)x.gacc;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) G[(int64_t)a * nq + c] += ga[a];
+        for (int a = 0; a < 3; ++a) G[(int64_t)a * rs + r] += ga[a];
     }
     if (zh) {
         for (int fc = 5; fc >= 0; --fc) {
@@ -642,7 +647,7 @@ LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, con
         else if constexpr (SPECIAL) lam[l] = L[e + (int64_t)opp(l) * p.ps];
         else lam[l] = ld_stream<NTE>(own(opp(l)) + kk);
     }
-    adj_core<P, RLX>(p, x, c, m, SPECIAL && (m & M_ZH_ANY), i, j, k, lam, mu);
+    adj_core<P, RLX>(p, x, c, c, nq, m, SPECIAL && (m & M_ZH_ANY), i, j, k, lam, mu);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         if constexpr (ODD) *loc(l) = mu[l];
@@ -690,7 +695,7 @@ __global__ __launch_bounds__(TILE) void k_adj_sp(StepParams p, AdjParams x) {
     if (n >= p.nf) return;
     TC* L = (TC*)x.lam;
     const uint32_t ps = (uint32_t)p.ps;
-    const uint32_t c = p.site[n];
+    const uint32_t c = ZH || x.seed || p.acc ? p.site[n] : 0u;  // uniform choice
     const uint32_t m = ZH ? p.mask[c] : M_FLUID;
     int i = 0, j = 0, k = 0;
     if ((m & M_ZH_ANY) || x.seed) decompose(c, p.d.ny, p.d.nzp, i, j, k);
@@ -705,7 +710,7 @@ __global__ __launch_bounds__(TILE) void k_adj_sp(StepParams p, AdjParams x) {
 #pragma unroll
         for (int l = 0; l < Q; ++l) lam[l] = ld_stream<NTE>(uniform_ptr(L + (int64_t)opp(l) * ps) + n);
     }
-    adj_core<P, RLX>(p, x, c, m, ZH && (m & M_ZH_ANY) != 0, i, j, k, lam, mu);
+    adj_core<P, RLX>(p, x, c, n, ps, m, ZH && (m & M_ZH_ANY) != 0, i, j, k, lam, mu);
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         if constexpr (ODD) L[loc[l]] = mu[l];
@@ -734,7 +739,7 @@ __global__ __launch_bounds__(TILE) void k_adj_sp_pull(StepParams p, AdjParams x)
     const TC* L = (const TC*)x.lam;
     TC* L2 = (TC*)x.lam2;
     const uint32_t ps = (uint32_t)p.ps;
-    const uint32_t c = p.site[n];
+    const uint32_t c = ZH || x.seed || p.acc ? p.site[n] : 0u;  // uniform choice
     const uint32_t m = ZH ? p.mask[c] : M_FLUID;
     int i = 0, j = 0, k = 0;
     if ((m & M_ZH_ANY) || x.seed) decompose(c, p.d.ny, p.d.nzp, i, j, k);
@@ -744,7 +749,7 @@ __global__ __launch_bounds__(TILE) void k_adj_sp_pull(StepParams p, AdjParams x)
     sparse_locs(p, n, loc);
 #pragma unroll
     for (int l = 0; l < Q; ++l) lam[opp(l)] = L[loc[l]];
-    adj_core<P, RLX>(p, x, c, m, ZH && (m & M_ZH_ANY) != 0, i, j, k, lam, mu);
+    adj_core<P, RLX>(p, x, c, n, ps, m, ZH && (m & M_ZH_ANY) != 0, i, j, k, lam, mu);
 #pragma unroll
     for (int l = 0; l < Q; ++l) st_stream<NTE>(uniform_ptr(L2 + (int64_t)l * ps) + n, mu[l]);
 }

@@ -758,8 +758,8 @@ static int dalloc(lb_solver* s, void** p, size_t n) {
     return LB_OK;
 }
 
-// An optional array (the second population array of the pull steps): null,
-// and the steps run in place, if the device cannot hold it.
+// An optional array (the second population or cotangent array of the pull
+// steps): null, and the steps run in place, if the device cannot hold it.
 static void dalloc_optional(lb_solver* s, void** p, size_t n) {
     if (hipMalloc(p, n) != hipSuccess) {
         *p = nullptr;
@@ -1734,12 +1734,10 @@ static int fp16_guard(lb_solver* s) {
 // `n` steps with the solver's current launch configuration; the last one
 // stores den/vel if `store_last`. FP16 storage runs its guard every
 // f16_check steps.
-// In the sparse layout, odd steps are pull steps (k_step_sp_pull) when the
-// second array is there: each leaves the state an even step leaves, in the
-// other array, which then becomes s->f; the parity stays odd.
-// One step from the state in s->f, whose parity is `odd` (updated): a pull
-// step into s->f2, which then becomes s->f, where pull steps are active and
-// the state is in the form an even step leaves; else an in-place AA step.
+// One step from the state in s->f, whose parity is `odd` (updated). Where
+// pull steps are active and the second array is there, an odd step is a pull
+// step: it leaves the state an even step leaves, in s->f2, which then becomes
+// s->f, and the parity stays odd. Otherwise an in-place AA step.
 static void one_step(lb_solver* s, StepParams& p, int& odd) {
     if (odd && s->pull_active && s->f2) {
         p.f2 = s->f2;
@@ -2459,10 +2457,11 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     // Pull-form adjoint steps (sparse layout) from 1 GiB of cotangents, like
     // the forward pull steps: on the 768x384x384 medium the reverse pass took
     // 5.26 ms per step in FP32 (10.39 in FP64) against 5.58 (10.99) in
-    // place; on the 128x64x64 channel, whose arrays stay in the caches, the
-    // in-place steps are faster (22.5 against 24.3 us per adjoint step).
+    // place, both with the record still indexed by cell; on the 128x64x64
+    // channel, whose arrays stay in the caches, the in-place steps are faster
+    // (22.5 against 24.3 us per adjoint step).
     const size_t lbytes = (size_t)s->fsize * tcs;
-    const bool want_lpull = s->tab && (s->adj_pull >= 0 ? s->adj_pull != 0 : lbytes >= ((size_t)1 << 30));
+    const bool want_lpull = s->sparse && (s->adj_pull >= 0 ? s->adj_pull != 0 : lbytes >= ((size_t)1 << 30));
     if (want_lpull && !s->lam2) dalloc_optional(s, &s->lam2, lbytes);
     const bool lpull = want_lpull && s->lam2;
     const size_t rn = rec_row(s);

@@ -254,10 +254,10 @@ int lb_adj_forward(lb_solver *s, uint32_t steps, uint32_t interval, int face, in
  * respect to the per-site acceleration field (f64 [nx][ny][nz][3]) and to the
  * velocity data of Zou-He face `face` (f64, face layout x 3; face < 0: none).
  * Each checkpoint segment is recomputed forward, recording (rho, u*) per site
- * and step, then stepped back with the adjoint AA step (in the sparse layout
- * from 1 GiB of cotangents, its pull form, which writes a second cotangent
- * array; LB_ADJ_PULL=0/1 in the environment decides instead; the results
- * are identical). The last segment uses
+ * and step, then stepped back with the adjoint AA step (from 1 GiB of
+ * cotangents its pull form, which writes a second cotangent array;
+ * LB_ADJ_PULL=0/1 in the environment decides instead; the results are
+ * identical). The last segment uses
  * the forward pass's record instead, unless a call since then changed an input
  * or an earlier reverse pass overwrote it (interval < steps). With interval >=
  * steps nothing is recomputed. The population array is left in its state after

@@ -288,7 +288,7 @@ LBD void ladd_apply(const StepParams& p, uint32_t c, uint32_t m, T* f) {
 
 struct AdjParams {
     void* lam;          // cotangent array, population layout of f, compute type
-    void* lam2;         // sparse layout, pull form: the array the step writes (null: in place)
+    void* lam2;         // pull form: the array the step writes (null: in place)
     const void* rec;    // (rho, u*) recorded by the forward recompute of this step
     void* gacc;         // accumulated acceleration cotangent, compute type, 3 rows indexed like rec
     double* gface;      // accumulated face-velocity cotangent (face of gface_id), face layout x 3

@@ -715,7 +715,7 @@ struct lb_solver {
     void* ckpt = nullptr;  // (nseg + 1) states: segment starts and the final state
     size_t ckpt_bytes = 0;
     void* lam = nullptr;
-    void* lam2 = nullptr;  // sparse layout: the second cotangent array of the pull-form adjoint steps, or null
+    void* lam2 = nullptr;  // the second cotangent array of the pull-form adjoint steps, or null
     int adj_pull = -1;     // pull-form adjoint steps: 1 on, 0 off, -1 from 1 GiB of cotangents (LB_ADJ_PULL sets 1 or 0)
     std::vector<uint8_t> ckpt_odd;  // next_odd of each checkpoint
     void* rec = nullptr;
@@ -2454,14 +2454,14 @@ static int adj_backward(lb_solver* s, int gface_id, double* grad_acc, double* gr
     const size_t fbytes = (size_t)s->fsize * ts_size(s->prec);
     int rc;
     if (!s->lam && (rc = dalloc(s, &s->lam, (size_t)s->fsize * tcs))) return rc;
-    // Pull-form adjoint steps (sparse layout) from 1 GiB of cotangents, like
-    // the forward pull steps: on the 768x384x384 medium the reverse pass took
-    // 5.26 ms per step in FP32 (10.39 in FP64) against 5.58 (10.99) in
+    // Pull-form adjoint steps from 1 GiB of cotangents, like the sparse
+    // layout's forward pull steps: on the 768x384x384 medium the reverse pass
+    // took 5.26 ms per step in FP32 (10.39 in FP64) against 5.58 (10.99) in
     // place, both with the record still indexed by cell; on the 128x64x64
     // channel, whose arrays stay in the caches, the in-place steps are faster
     // (22.5 against 24.3 us per adjoint step).
     const size_t lbytes = (size_t)s->fsize * tcs;
-    const bool want_lpull = s->sparse && (s->adj_pull >= 0 ? s->adj_pull != 0 : lbytes >= ((size_t)1 << 30));
+    const bool want_lpull = s->adj_pull >= 0 ? s->adj_pull != 0 : lbytes >= ((size_t)1 << 30);
     if (want_lpull && !s->lam2) dalloc_optional(s, &s->lam2, lbytes);
     const bool lpull = want_lpull && s->lam2;
     const size_t rn = rec_row(s);

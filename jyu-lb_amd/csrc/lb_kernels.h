@@ -612,8 +612,10 @@ LBD void adj_core(const StepParams& p, const AdjParams& x, const uint32_t c, con
 
 // One site of the adjoint step. MODE_PLAIN / MODE_BULK address the
 // cotangents like the bulk step kernel (wave-uniform row bases, bounce-back
-// links from the link mask); MODE_SPECIAL resolves every link.
-template <int P, int RLX, bool ODD, int MODE>
+// links from the link mask); MODE_SPECIAL resolves every link. PULL (with
+// ODD): the pull form of the sparse layout's k_adj_sp_pull, the odd step's
+// gathers from x.lam and the even step's stores into x.lam2.
+template <int P, int RLX, bool ODD, int MODE, bool PULL = false>
 LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, const uint32_t m) {
     using TC = typename Var<P>::TC;
     constexpr bool SPECIAL = MODE == MODE_SPECIAL;
@@ -648,6 +650,15 @@ LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, con
         else lam[l] = ld_stream<NTE>(own(opp(l)) + kk);
     }
     adj_core<P, RLX>(p, x, c, c, nq, m, SPECIAL && (m & M_ZH_ANY), i, j, k, lam, mu);
+    if constexpr (PULL) {
+        TC* L2 = (TC*)x.lam2;
+#pragma unroll
+        for (int l = 0; l < Q; ++l) {
+            if constexpr (SPECIAL) L2[e + (int64_t)l * p.ps] = mu[l];
+            else st_stream<NTE>(uniform_ptr(L2 + (int64_t)row * p.rs + (int64_t)l * p.ps) + kk, mu[l]);
+        }
+        return;
+    }
 #pragma unroll
     for (int l = 0; l < Q; ++l) {
         if constexpr (ODD) *loc(l) = mu[l];
@@ -657,7 +668,9 @@ LBD void adj_site(const StepParams& p, const AdjParams& x, const uint32_t c, con
 }
 
 // Adjoint step over the non-special fluid sites (same tiling as k_step).
-template <int P, int RLX, bool ODD>
+// PULL (with ODD): the pull form, at either parity; the bulk and special-site
+// launches both read x.lam and write x.lam2.
+template <int P, int RLX, bool ODD, bool PULL = false>
 __global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
     const uint32_t tile = tile_index(p);
     if (tile >= p.ntiles) return;
@@ -665,21 +678,21 @@ __global__ __launch_bounds__(TILE) void k_adj(StepParams p, AdjParams x) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(c >> 6);
     if (w >= p.nwaves) return;
     if (LB_PLAIN_PATH && p.wclass[w] == 0) {
-        adj_site<P, RLX, ODD, MODE_PLAIN>(p, x, c, M_FLUID);
+        adj_site<P, RLX, ODD, MODE_PLAIN, PULL>(p, x, c, M_FLUID);
         return;
     }
     const uint32_t m = p.mask[c];
     if ((m & (M_FLUID | special_bits<ODD>())) != M_FLUID) return;
-    adj_site<P, RLX, ODD, MODE_BULK>(p, x, c, m);
+    adj_site<P, RLX, ODD, MODE_BULK, PULL>(p, x, c, m);
 }
 
 // Adjoint step over the special sites (compact list).
-template <int P, int RLX, bool ODD>
+template <int P, int RLX, bool ODD, bool PULL = false>
 __global__ __launch_bounds__(TILE) void k_adj_special(StepParams p, AdjParams x) {
     const uint32_t t = blockIdx.x * TILE + threadIdx.x;
     if (t >= (ODD ? p.nspecial : p.nzh)) return;
     const uint32_t c = p.special[t];
-    adj_site<P, RLX, ODD, MODE_SPECIAL>(p, x, c, p.mask[c]);
+    adj_site<P, RLX, ODD, MODE_SPECIAL, PULL>(p, x, c, p.mask[c]);
 }
 
 // Adjoint step of the sparse layout, one thread per compact fluid site: the
@@ -931,6 +944,11 @@ static void launch_adj(const StepParams& ps, const AdjParams& x, bool odd, hipSt
             if (odd) k_adj_sp<P, RLX, true, false><<<grid, TILE, 0, st>>>(p, x);
             else k_adj_sp<P, RLX, false, false><<<grid, TILE, 0, st>>>(p, x);
         }
+        return;
+    }
+    if (x.lam2) {  // pull form: the odd step's site classes at either parity
+        k_adj<P, RLX, true, true><<<grid, TILE, 0, st>>>(p, x);
+        if (p.nspecial) k_adj_special<P, RLX, true, true><<<(p.nspecial + TILE - 1) / TILE, TILE, 0, st>>>(p, x);
         return;
     }
     if (odd) k_adj<P, RLX, true><<<grid, TILE, 0, st>>>(p, x);

@@ -398,21 +398,24 @@ def test_adjoint_identical_in_sparse_layout(precision, rlx, steps_bc):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("pre_steps", [0, 1], ids=["from_odd", "from_even"])
 @pytest.mark.parametrize("layout", ["sparse", "rows"])
-def test_adjoint_pull_forms_identical(monkeypatch, precision, pre_steps, layout):
+@pytest.mark.parametrize("steps_bc", [(PERIODIC, PERIODIC), (WALL, SYMMETRIC)], ids=["periodic", "symmetric"])
+def test_adjoint_pull_forms_identical(monkeypatch, precision, pre_steps, layout, steps_bc):
     """The forward and recompute passes in place or as pull steps
-    (LB_SP_PULL), and in the sparse layout the adjoint steps in place or in
-    pull form (LB_ADJ_PULL, a second cotangent array), from either parity.
+    (LB_SP_PULL), and the adjoint steps in place or in pull form
+    (LB_ADJ_PULL, a second cotangent array), from either parity, with
+    periodic or wall and symmetric faces beside the Zou-He inlet and outlet.
     Objective and both gradients are bit-identical in every combination, for
     the flux objective with segments of unequal length (30 steps, checkpoint
     every 7) and for the dissipation objective."""
     from jyulb import adjoint
     out, diss = {}, {}
     for sp in ("0", "1"):
-        for ap in (("0", "1") if layout == "sparse" else ("0",)):
+        for ap in ("0", "1"):
             monkeypatch.setenv("LB_SP_PULL", sp)
             monkeypatch.setenv("LB_ADJ_PULL", ap)
-            print(f"pull forms {layout} {precision} pre {pre_steps} LB_SP_PULL={sp} LB_ADJ_PULL={ap}", flush=True)
-            s, prof, amp = make_channel(TRT, precision, steps_bc=(PERIODIC, PERIODIC), seed=8, layout=layout)
+            print(f"pull forms {layout} {steps_bc} {precision} pre {pre_steps} LB_SP_PULL={sp} LB_ADJ_PULL={ap}",
+                  flush=True)
+            s, prof, amp = make_channel(TRT, precision, steps_bc=steps_bc, seed=8, layout=layout)
             assert s.engine.pull_steps() == (sp == "1")
             if pre_steps:
                 s.engine.evolve(pre_steps)

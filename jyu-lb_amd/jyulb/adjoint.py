@@ -23,6 +23,8 @@ interval >= steps nothing is recomputed, at 16 bytes per site and step in FP32
 the start are the Solver's current state. Walls must be stationary; the
 arithmetic is FP64 or FP32 (the Solver's precision).
 """
+import os
+
 import numpy as np
 
 from .defs import FACE_X1, X, FIXED_DEN, FIXED_VEL
@@ -52,9 +54,9 @@ def checkpoint_interval(solver, steps, budget_bytes=None, objective="flux"):
     choice, and i = steps (one segment) recomputes nothing. Memory for
     interval i: (ceil(steps / i) + 1) population arrays (19 values per site)
     for the checkpoints, min(i, steps) x 4 values per site for the (rho, u*)
-    record, one population array of cotangents (two in the sparse layout,
-    whose adjoint steps from 1 GiB of cotangents write the other array; the
-    budget counts the second array at every size) and 3 values per site of the
+    record, one population array of cotangents (two where the adjoint steps
+    take their pull form: from 1 GiB of cotangents, or as LB_ADJ_PULL says)
+    and 3 values per site of the
     acceleration cotangent, plus (Q + 4) values per site of seeds for the
     dissipation objective; nx * ny * round_up(nz, 64) sites and 8 bytes per
     value in FP64, 4 in FP32. Raises MemoryError if even the smallest
@@ -66,7 +68,15 @@ def checkpoint_interval(solver, steps, budget_bytes=None, objective="flux"):
     nq = int(nx) * int(ny) * ((int(nz) + 63) // 64 * 64)
     vb = 8 if eng.precision == LB_FP64 else 4
     fb, rb = 19 * nq * vb, 4 * nq * vb
-    extra = (2 if eng.layout() == "sparse" else 1) * fb + 3 * nq * vb
+    env = os.environ.get('LB_ADJ_PULL')
+    if env is not None:
+        try:
+            pull = int(env) != 0
+        except ValueError:
+            pull = False
+    else:
+        pull = fb >= 1 << 30
+    extra = (2 if pull else 1) * fb + 3 * nq * vb
     extra += 23 * nq * vb if objective == "dissipation" else 0
     steps = int(steps)
     if steps < 1:

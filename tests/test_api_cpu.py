@@ -112,18 +112,20 @@ def test_package_exports():
     assert callable(adjoint.flux_gradient)
 
 
-def test_checkpoint_interval_choice():
+def test_checkpoint_interval_choice(monkeypatch):
     """interval='auto' of the gradient entry points: the fewest checkpoint
     segments whose memory fits the budget. Lattice 10 x 4 x 70 has
     10 * 4 * 128 = 5120 padded sites: a population array is 778,240 bytes in
     FP64, one step of (rho, u*) record 163,840, and the reverse pass holds a
     cotangent array and 3 acceleration cotangents per site (901,120 bytes),
-    and in the sparse layout a second cotangent array."""
+    and a second cotangent array where the adjoint steps take their pull
+    form (from 1 GiB of cotangents, or as LB_ADJ_PULL says)."""
     from types import SimpleNamespace
     from jyulb.adjoint import checkpoint_interval
     from jyulb._native import LB_FP64, LB_FP32_SHIFTED
-    s64 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64, layout=lambda: "rows"))
-    s32 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP32_SHIFTED, layout=lambda: "rows"))
+    monkeypatch.delenv("LB_ADJ_PULL", raising=False)
+    s64 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64))
+    s32 = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP32_SHIFTED))
     fb, rb, extra = 778_240, 163_840, 901_120
     one = 2 * fb + 100 * rb + extra  # one segment: 18,841,600 bytes
     assert checkpoint_interval(s64, 100, one) == 100
@@ -144,10 +146,21 @@ def test_checkpoint_interval_choice():
         checkpoint_interval(s64, 100, 6 * fb + 20 * rb + extra - 1)
     with pytest.raises(ValueError):
         checkpoint_interval(s64, 0, 1 << 40)
-    # the sparse layout's adjoint steps hold a second cotangent array
-    sp = SimpleNamespace(engine=SimpleNamespace(lat=(10, 4, 70), precision=LB_FP64, layout=lambda: "sparse"))
-    assert checkpoint_interval(sp, 100, one + fb) == 100
-    assert checkpoint_interval(sp, 100, one + fb - 1) == 50
+    # pull-form adjoint steps hold a second cotangent array: forced here ...
+    monkeypatch.setenv("LB_ADJ_PULL", "1")
+    assert checkpoint_interval(s64, 100, one + fb) == 100
+    assert checkpoint_interval(s64, 100, one + fb - 1) == 50
+    monkeypatch.delenv("LB_ADJ_PULL")
+    # ... and by default from 1 GiB of cotangents: 512^3 in FP64, 134,217,728
+    # sites, 20,401,094,656 bytes per array
+    big = SimpleNamespace(engine=SimpleNamespace(lat=(512, 512, 512), precision=LB_FP64))
+    bfb, brb = (19 * 8) << 27, (4 * 8) << 27
+    bextra = 2 * bfb + ((3 * 8) << 27)
+    assert checkpoint_interval(big, 1, 2 * bfb + brb + bextra) == 1
+    with pytest.raises(MemoryError):
+        checkpoint_interval(big, 1, 2 * bfb + brb + bextra - 1)
+    monkeypatch.setenv("LB_ADJ_PULL", "0")
+    assert checkpoint_interval(big, 1, 2 * bfb + brb + bextra - bfb) == 1
 
 
 @pytest.mark.parametrize("lat", [(1024, 1024, 2048), (1024, 1024, 2049), (4096, 1024, 1024)])

@@ -49,7 +49,7 @@ extern "C" {
 #define LB_LAYOUT_SPARSE 3      /* fluid sites only: f[l][n] over the compacted fluid sites, with a table of
                                    the 18 odd-step locations per site (the fluid-only AA arrays of
                                    include/kernel/kernel.cpp:227-271), built on the device and read
-                                   as 5-bit codes against per-wave bases; allocated by lb_finalize */
+                                   as 4-bit codes against per-wave bases; allocated by lb_finalize */
 #define LB_LAYOUT_AUTO 4        /* chosen at the first lb_finalize: SPARSE when at least 5% of the cells are
                                    solid (LB_AUTO_SPARSE_SOLID in the environment sets the fraction), ROWS
                                    otherwise; results are identical in every layout */

@@ -139,16 +139,19 @@ LBD void decompose(uint32_t c, int ny, int nzp, int& i, int& j, int& k) {
 // step kernel
 // ---------------------------------------------------------------------------
 // Sparse-layout location codes of the odd step (see step_sparse): SP_BITS
-// bits per link, SP_PER_WORD links per 32-bit word, SP_WORDS words per site. A
-// code below SP_ALT is the offset of the location from the wave's base for
-// the link plus the site's lane, SP_ALT names the wave's alternate location
-// for the link, SP_BOUNCE the site's own slot l, SP_FAR a location read from
-// the 32-bit table. Per wave: SP_WAVE_WORDS words, the 18 bases, then the 18
-// alternates.
-constexpr int SP_BITS = 5, SP_PER_WORD = 6, SP_WORDS = 3;
+// bits per link, SP_PER_WORD links per 32-bit word. Per site SP_WORDS words
+// (link slots 0-15) and one byte (slots 16 and 17): SP_CODE_BYTES bytes, in
+// SP_WORDS rows of 32-bit words and a row of bytes. A code below SP_ALT is the
+// offset of the location from the wave's base for the link plus the site's
+// lane, SP_ALT names the wave's alternate location for the link, SP_BOUNCE
+// the site's own slot l, SP_FAR a location read from the 32-bit table. Per
+// wave: SP_WAVE_WORDS words, the 18 bases, then the 18 alternates.
+constexpr int SP_BITS = 4, SP_PER_WORD = 8, SP_WORDS = 2;
+constexpr int SP_CODE_BYTES = 4 * SP_WORDS + 1;
 constexpr int SP_WAVE_WORDS = 36;
-constexpr uint32_t SP_FAR = 31u, SP_BOUNCE = 30u, SP_ALT = 29u;
-static_assert(SP_PER_WORD * SP_BITS <= 32 && SP_PER_WORD * SP_WORDS == 18, "code packing");
+constexpr uint32_t SP_FAR = 15u, SP_BOUNCE = 14u, SP_ALT = 13u;
+static_assert(SP_PER_WORD * SP_BITS == 32 && SP_PER_WORD * SP_WORDS + 8 / SP_BITS == 18, "code packing");
+// code of link slot sl from w[0..SP_WORDS-1] and the site's byte in w[SP_WORDS]
 LBD uint32_t sp_code(const uint32_t* w, int sl) {
     return (w[sl / SP_PER_WORD] >> (SP_BITS * (sl % SP_PER_WORD))) & ((1u << SP_BITS) - 1u);
 }
@@ -201,7 +204,8 @@ struct StepParams {
     const uint32_t* cidx;
     const uint32_t* site;
     const uint32_t* tab;
-    const uint32_t* tabc;   // compact form of tab (see step_sparse): SP_WORDS rows of nf words
+    const uint32_t* tabc;   // compact form of tab (see step_sparse): SP_WORDS rows of ps words
+    const uint8_t* tabb;    // ... and its row of ps bytes (link slots 16 and 17), after them
     const uint32_t* tabw;   // its base and alternate locations, SP_WAVE_WORDS per wave of 64 compact sites
     uint32_t nf;  // fluid sites = compact sites the step processes
     // dissipation objective (k_diss): per site 19 population seeds, 3

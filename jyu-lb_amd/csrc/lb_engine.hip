@@ -666,7 +666,7 @@ struct lb_solver {
     uint32_t* cidx = nullptr;
     uint32_t* site = nullptr;
     uint32_t* tab = nullptr;
-    uint32_t* tabc = nullptr;  // SP_WORDS words per compact site (k_sparse_pack)
+    uint32_t* tabc = nullptr;  // SP_CODE_BYTES per compact site: SP_WORDS rows of words, a row of bytes
     uint32_t* tabw = nullptr;
     void* f2 = nullptr;   // sparse layout: the second population array of the pull steps (run_steps), or null
     int sp_pull = -1;     // pull steps: 1 on, 0 off, -1 by sp_pull_on / the launch tuner (LB_SP_PULL sets 1 or 0)
@@ -820,6 +820,7 @@ static StepParams params(const lb_solver* s) {
         p.site = s->site;
         p.tab = s->tab;
         p.tabc = s->tabc;
+        p.tabb = (const uint8_t*)(s->tabc + (size_t)SP_WORDS * s->nfp);
         p.tabw = s->tabw;
         p.nf = (uint32_t)s->nfluid;
     }
@@ -1058,7 +1059,7 @@ __global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_
     const uint32_t w = blockIdx.x, lane = threadIdx.x;
     const uint32_t n = w * 64 + lane;  // < nfp: the grid is nfp / 64 waves
     const bool live = n < nf;
-    uint32_t word[SP_WORDS] = {};
+    uint32_t word[SP_WORDS + 1] = {};
     for (int l = 0; l < Q; ++l) {
         if (l == REST) continue;
         const uint32_t sl = link_slot(l);
@@ -1098,6 +1099,7 @@ __global__ __launch_bounds__(64) void k_sparse_pack(const uint32_t* tab, uint32_
         }
     }
     for (int q = 0; q < SP_WORDS; ++q) tabc[(size_t)q * nfp + n] = word[q];
+    ((uint8_t*)(tabc + (size_t)SP_WORDS * nfp))[n] = (uint8_t)word[SP_WORDS];
 }
 
 static int count_mask(lb_solver* s, int64_t* nfluid, int64_t* nmoving) {
@@ -1529,13 +1531,13 @@ static int build_sparse(lb_solver* s) {
         dfree(s, &s->snap, Q * o * ts);
         dfree(s, (void**)&s->site, o * 4);
         dfree(s, (void**)&s->tab, (Q - 1) * o * 4);
-        dfree(s, (void**)&s->tabc, SP_WORDS * o * 4);
+        dfree(s, (void**)&s->tabc, SP_CODE_BYTES * o);
         dfree(s, (void**)&s->tabw, SP_WAVE_WORDS * (o / 64) * 4);
         dfree(s, &s->f2, Q * o * ts);
         s->nfp = 0;
         if ((rc = dalloc(s, &s->f, Q * nfp * ts)) || (rc = dalloc(s, (void**)&s->site, nfp * 4)) ||
             (rc = dalloc(s, (void**)&s->tab, (Q - 1) * nfp * 4)) ||
-            (rc = dalloc(s, (void**)&s->tabc, SP_WORDS * nfp * 4)) ||
+            (rc = dalloc(s, (void**)&s->tabc, SP_CODE_BYTES * nfp)) ||
             (rc = dalloc(s, (void**)&s->tabw, SP_WAVE_WORDS * (nfp / 64) * 4)))
             return rc;
         if (sp_pull_on(s, Q * nfp * ts)) dalloc_optional(s, &s->f2, Q * nfp * ts);

@@ -363,15 +363,16 @@ __global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
 //
 // The step reads the table in a compact form (k_sparse_pack). Per wave of 64
 // compact sites and link: a base and an alternate location (32 bits each).
-// Per site and link a 5-bit code: below SP_ALT, the location is base + lane
+// Per site and link a 4-bit code: below SP_ALT, the location is base + lane
 // + code (in a run of fluid sites the upstream locations advance with the
 // lane, so the codes stay small); SP_ALT, the wave's alternate location (it
 // takes the one outlier of most waves: the lane whose link wraps around a
 // periodic face at the end of a row); SP_BOUNCE, the own slot l; SP_FAR, a
-// location read from the 32-bit table (links of about 1% of the waves on a
-// random-sphere medium). A site's 18 codes are three 32-bit words, six links
-// each, in rows of ps words: three loads of whole cache lines per wave, 12
-// bytes per site (8-bit codes, 18 bytes, measured 1-4% slower).
+// location read from the 32-bit table (0.2% of the links on a random-sphere
+// medium). A site's 18 codes are two 32-bit words of eight links and a byte
+// of two, in two rows of ps words and a row of ps bytes: three loads per
+// wave, 9 bytes per site (5-bit codes in three words, 12 bytes, measured
+// 0.7% slower in FP64 and 2% in FP32; 8-bit codes, 18 bytes, slower still).
 
 // Odd-step locations of the 18 links of compact site n (loc[REST]: own slot).
 // The codes are decoded with selects, so that all 18 population loads after
@@ -380,7 +381,7 @@ __global__ __launch_bounds__(TILE) void k_diss_special(StepParams p) {
 LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
     const uint32_t ps = (uint32_t)p.ps;
     const uint32_t* B = p.tabw + (size_t)__builtin_amdgcn_readfirstlane(n >> 6) * SP_WAVE_WORDS;
-    uint32_t base[Q - 1], alt[Q - 1], w[SP_WORDS];
+    uint32_t base[Q - 1], alt[Q - 1], w[SP_WORDS + 1];
 #pragma unroll
     for (int sl = 0; sl < Q - 1; ++sl) {
         base[sl] = B[sl];
@@ -391,6 +392,7 @@ LBD void sparse_locs(const StepParams& p, const uint32_t n, uint32_t* loc) {
     for (int sl = 0; sl < Q - 1; ++sl) asm volatile("" ::"s"(base[sl]), "s"(alt[sl]));
 #pragma unroll
     for (int q = 0; q < SP_WORDS; ++q) w[q] = p.tabc[(size_t)q * ps + n];
+    w[SP_WORDS] = p.tabb[n];
     const uint32_t lane = n & 63u;  // waves start at multiples of 64
     uint32_t far = 0;
 #pragma unroll

@@ -214,9 +214,9 @@ def test_sparse_device_memory_follows_fluid_sites():
     ncell = 64 * 32 * 64
     # rows: 19 f64 per cell; sparse: per padded fluid site 19 f64 (one array:
     # pull steps only from 1 GiB of populations), its site index, the 18 u32
-    # locations and their 5-bit codes (three u32), per wave of 64 the 18 u32
-    # bases and 18 alternates, and a compact index per cell
-    sparse = (19 * 8 + 4 + 18 * 4 + 12) * nfp + 36 * 4 * (nfp // 64) + 4 * ncell
+    # locations and their 4-bit codes (two u32 and a byte), per wave of 64 the
+    # 18 u32 bases and 18 alternates, and a compact index per cell
+    sparse = (19 * 8 + 4 + 18 * 4 + 9) * nfp + 36 * 4 * (nfp // 64) + 4 * ncell
     assert b_rows - b_sp == 19 * 8 * ncell - sparse
 
 

@@ -153,16 +153,20 @@ def test_gpu_fp64_bit_exact_with_step_tile(name, tile, monkeypatch):
 def test_gpu_fp64_bit_exact_with_launch_tuning(name, monkeypatch):
     """The first long evolve call times its own steps under each candidate
     launch configuration and keeps the fastest; forced on for these small
-    lattices, the FP64 results still equal the reference's bit for bit."""
+    lattices, the FP64 results still equal the reference's bit for bit, in
+    the rows layout and in the sparse layout (whose candidates are the tile
+    orders)."""
     monkeypatch.setenv("LB_AUTOTUNE", "1")
     monkeypatch.setenv("LB_AUTOTUNE_MIN_CELLS", "0")
     path = os.path.join(os.path.dirname(__file__), "golden", name + ".npz")
     test_gpu_fp64_bit_exact(path, 0)
+    test_gpu_fp64_bit_exact(path, 3)
 
 
+@pytest.mark.parametrize("layout", ["auto", "sparse"])
 @pytest.mark.parametrize("precision", ["fp32", "fp32_storage", "fp16"])
 @pytest.mark.parametrize("name", ["trt_long", "ex_couette"])
-def test_launch_tuning_changes_no_bit(name, precision, monkeypatch):
+def test_launch_tuning_changes_no_bit(name, precision, layout, monkeypatch):
     z = np.load(os.path.join(os.path.dirname(__file__), "golden", name + ".npz"))
     sc = json.loads(str(z["scenario"]))
     Solver = _solver_cls()
@@ -170,7 +174,8 @@ def test_launch_tuning_changes_no_bit(name, precision, monkeypatch):
     outs = []
     for env in ("0", "1"):
         monkeypatch.setenv("LB_AUTOTUNE", env)
-        outs.append(scenarios.run_scenario(lambda *a: Solver(*a, precision=precision), sc, record_f=True))
+        outs.append(scenarios.run_scenario(lambda *a: Solver(*a, precision=precision, layout=layout), sc,
+                                           record_f=True))
     for k in outs[0]:
         np.testing.assert_array_equal(outs[1][k], outs[0][k], err_msg=f"{name} {precision} {k}")
 

@@ -1213,7 +1213,7 @@ int lb_device_count(void) {
 }
 
 // Lattices below this many cells are not launch-tuned by default.
-constexpr uint64_t SMALL_SPARSE_CELLS = 1ull << 22;
+constexpr uint64_t TUNE_MIN_CELLS = 1ull << 22;
 
 // Launch defaults of a population layout, then the environment overrides.
 static void layout_defaults(lb_solver* s, int layout) {
@@ -1240,7 +1240,7 @@ static void layout_defaults(lb_solver* s, int layout) {
     // 0.0273 / 0.0265; 256x128x128 FP32 0.0944 / 0.0984, FP16 0.0627 /
     // 0.0650 (tuned there, with the XCD orders among the candidates);
     // 384x192x192 FP32 0.359 / 0.345.
-    if (sparse && s->prec != LB_FP64 && s->nq < SMALL_SPARSE_CELLS) {
+    if (sparse && s->prec != LB_FP64 && s->nq < TUNE_MIN_CELLS) {
         s->xcd_cells = 0;
         s->xcd_chunk = 0;
     }
@@ -1352,7 +1352,7 @@ int lb_create(const lb_config* cfg, lb_solver** out) {
     // LB_AUTOTUNE_MIN_CELLS changes the size, LB_AUTOTUNE_VERBOSE=1 prints
     // the timings.
     {
-        uint64_t min_cells = SMALL_SPARSE_CELLS;
+        uint64_t min_cells = TUNE_MIN_CELLS;
         if (const char* env = getenv("LB_AUTOTUNE_MIN_CELLS")) min_cells = strtoull(env, nullptr, 10);
         const char* at = getenv("LB_AUTOTUNE");
         s->tune = (!at || strtol(at, nullptr, 10) != 0) && !getenv("LB_STEP_TILE") && !getenv("LB_STEP_LDS_EVEN") &&

@@ -3,9 +3,10 @@
 Headline workload: the porous medium of 768 x 384 x 384 cells (113M cells,
 90.6M fluid sites): seeded random spheres of radius 6..12 up to 20 % solid
 (jyulb.geometry.porous_spheres, seed 5), D3Q19 TRT (magic 3/16), tau = 0.6,
-gravity gx = 1e-6, periodic on every face. A "step" is one fused AA
-stream+collide pass over the lattice, with the populations resident in HBM,
-timed with HIP events on the stream the kernels run on. `value` is FP64 MLUPS
+gravity gx = 1e-6, periodic on every face. A "step" is one fused
+stream+collide pass over the lattice (an AA step in place, or a pull step into
+the second population array), with the populations resident in HBM, timed with
+HIP events on the stream the kernels run on. `value` is FP64 MLUPS
 (fluid-site updates) in the Solver's default layout ("auto", which picks the
 sparse fluid-only layout for this medium); FP32 on shifted populations, FP16
 storage / FP32 arithmetic and FP32 storage / FP64 arithmetic are in
